@@ -1,0 +1,120 @@
+/*
+ * spires_dev.h -- thin C-ABI device layer between the C++ host library and
+ * the gfx950 HIP kernels.
+ *
+ * The host (spires_amd/csrc/host) never includes HIP headers; it hands this
+ * layer plain host pointers and sizes.  Everything behind it lives in device
+ * memory (HBM) of the current HIP device and runs on one HIP stream per
+ * reservoir.  Functions return 0 on success, SPIRES_DEV_EALLOC on an
+ * allocation failure and SPIRES_DEV_EINTERNAL otherwise;
+ * spires_dev_last_error() gives the message.
+ *
+ * This layer has no counterpart in the reference (which is CPU-only); it
+ * replaces the bodies of step_reservoir / compute_output /
+ * train_output_* in src/reservoir.c:113-192, 244-280, 573-767.
+ */
+#ifndef SPIRES_DEV_H
+#define SPIRES_DEV_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SPIRES_DEV_EALLOC    1
+#define SPIRES_DEV_EINTERNAL 2
+#define SPIRES_DEV_EINVAL    3
+
+typedef struct spires_dev spires_dev;
+
+/* Per-model scalars.  Meaning by neuron model (index into np[]):
+ *  LIF_DISCRETE   0 V_0, 1 V_th, 2 leak, 3 bias
+ *  LIF_BIO        0 V_0, 1 V_th, 2 tau, 3 bias
+ *  FLIF_GL        0 V_th, 1 V_reset, 2 V_rest, 3 tau_m, 4 pow(dt,alpha), 5 bias, 6 t_ref
+ *  FLIF_CAPUTO    0 C_m, 1 g_l, 2 V_l, 3 V_th, 4 V_0, 5 kr, 6 t_ref, 7 initial t_prev
+ *  FLIF_DIFFUSIVE 0 V_th, 1 V_reset, 2 V_rest, 3 g_euler, 4 bias, 5 alpha,
+ *                 6 t_ref, 7 t_min, 8 t_max, 9 pow(alpha,2), 10 q,
+ *                 11 pow(t_min,1-q), 12 pow(t_max,1-q), 13 1/(1-q)          */
+typedef struct {
+    int32_t n, din, dout, micro;
+    int32_t neuron_type, synapse_type, backend;
+    double  gain, dt;
+    double  np[16];
+    int32_t L;                 /* GL ring length / Caputo mem_len          */
+    const double *coef;        /* GL c[0..L-1] / Caputo weights [L-2]      */
+    /* recurrent weights: CSR (row i = inputs of neuron i) or dense n*n   */
+    int64_t nnz;
+    const int64_t *row_ptr;
+    const int32_t *col;
+    const double  *val;
+    const double  *dense;      /* row-major n*n (dense backend)            */
+    /* synapse tables */
+    double  syn_decay;         /* PSC-homogeneous exp(-dt/tau_syn)         */
+    const double *edge_decay;  /* PSC-heterogeneous exp(-dt/tau_k) per edge (CSR order or n*n) */
+    int32_t frac_L;            /* fractional synapse memory length         */
+    const double *frac_coef;   /* [frac_L]                                  */
+    double  frac_dt_alpha, frac_tau;
+    int32_t me_n;              /* multi-exponential component count        */
+    const double *me_decay;    /* [me_n] exp(-dt/tau_c)                     */
+    const double *me_c;        /* [me_n] mixing weights                     */
+    const double *W_in;        /* [n x din]                                 */
+    const double *W_out;       /* [dout x n]                                */
+    const uint64_t *rng_state; /* diffusive: initial per-neuron RNG state [n] */
+} spires_dev_desc;
+
+/* Execution modes of the reservoir step. */
+#define SPIRES_DEV_MODE_EXACT 0   /* FP64, bit-exact with the CPU reference order */
+#define SPIRES_DEV_MODE_FAST  1   /* FP32 time-blocked history contraction        */
+
+int  spires_dev_create(const spires_dev_desc *d, spires_dev **out);
+void spires_dev_destroy(spires_dev *g);
+/* Re-initialise neuron state (scalars and tables may change; synapse state
+ * is kept, as in the reference's reset). */
+int  spires_dev_reset_neurons(spires_dev *g, const spires_dev_desc *d);
+
+/* One macro step with input u[din] (NULL = zeros). Asynchronous. */
+int  spires_dev_step(spires_dev *g, const double *u);
+/* Resident input series: upload u[T x din] once. */
+int  spires_dev_upload_inputs(spires_dev *g, const double *u, size_t T);
+/* Run steps t0..t0+T-1 of the resident series.  flags: */
+#define SPIRES_DEV_REC_STATE  1u   /* write V after each step into the state record */
+#define SPIRES_DEV_REC_RASTER 2u   /* write spikes (u8) into the raster record        */
+#define SPIRES_DEV_REC_OUTPUT 4u   /* write y = W_out V into the output record        */
+/* Record buffers are (re)allocated to hold T rows starting at row 0. */
+int  spires_dev_run(spires_dev *g, size_t t0, size_t T, unsigned flags);
+/* Pre-size the record buffers (so a later run allocates nothing). */
+int  spires_dev_reserve_records(spires_dev *g, size_t rows, unsigned flags);
+int  spires_dev_read_records(spires_dev *g, size_t row0, size_t rows, double *states,
+                             uint8_t *raster, double *outputs);
+
+int  spires_dev_read_state(spires_dev *g, double *V);
+int  spires_dev_set_state(spires_dev *g, const double *V);
+int  spires_dev_read_spikes(spires_dev *g, double *S);
+int  spires_dev_compute_output(spires_dev *g, double *y);
+int  spires_dev_get_readout(spires_dev *g, double *W_out);
+int  spires_dev_set_readout(spires_dev *g, const double *W_out);
+
+/* Ridge on the recorded states rows [washout, T) against targets
+ * Y[(T - washout) x dout] (host): W_out = (X'X + lambda I)^-1 X'Y by FP64
+ * Cholesky with iterative refinement (LU if not positive definite).
+ * info_out (optional, 2 doubles): [0] 1 if LU was used, [1] final relative
+ * residual of the normal equations. */
+int  spires_dev_ridge(spires_dev *g, size_t washout, const double *Y, double lambda,
+                      double *info_out);
+/* Recursive least squares over T resident input steps with targets Y[T x dout]. */
+int  spires_dev_rls(spires_dev *g, const double *Y, size_t T, double delta, double lambda);
+/* Delta rule on the current state. */
+int  spires_dev_online(spires_dev *g, const double *target, double lr);
+
+int  spires_dev_set_mode(spires_dev *g, int mode, int block_k);
+int  spires_dev_sync(spires_dev *g);
+void *spires_dev_stream(spires_dev *g);
+size_t spires_dev_steps(const spires_dev *g);
+const char *spires_dev_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

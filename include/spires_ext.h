@@ -1,0 +1,94 @@
+/*
+ * spires_ext.h -- extensions of the spires C ABI for the MI355X engine.
+ *
+ * These entry points have no counterpart in the reference's include/spires.h.
+ * They exist for callers that need what the reference can only do through
+ * private structs (spoken_digit_recording.c reaches into them for rasters and
+ * weights) or cannot do at all: reservoirs built from caller-supplied
+ * weights, whole-run recording of states / spike rasters without per-step
+ * API round trips, device-resident inputs for timing, a ridge readout with a
+ * washout, and the choice between the FP64 exact and the FP32 fast step.
+ */
+#ifndef SPIRES_EXT_H
+#define SPIRES_EXT_H
+
+#include "spires.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Create from an explicit recurrent matrix in CSR form (row i lists the
+ * presynaptic inputs of neuron i, columns ascending), input weights
+ * W_in [N x Din] and optional W_out [Dout x N] (NULL = zeros).  No spectral
+ * rescaling and no rand() draws happen, except the per-edge tau draw of the
+ * PSC-heterogeneous synapse.  Zero-valued entries are dropped. */
+spires_status spires_ext_create_csr(const spires_reservoir_config *cfg, size_t nnz,
+                                    const int64_t *row_ptr, const int32_t *col_idx,
+                                    const double *values, const double *W_in,
+                                    const double *W_out, spires_reservoir **out_r);
+
+/* The weights spires_reservoir_create would build for cfg (same rand()
+ * draws: W_in, topology, synapse taus, spectral rescale, W_out), without
+ * creating a reservoir or touching the GPU.  Outputs are malloc'd; the caller
+ * free()s them.  Recurrent matrix as CSR with ascending columns. */
+spires_status spires_ext_build_weights(const spires_reservoir_config *cfg, size_t *nnz_out,
+                                       int64_t **row_ptr, int32_t **col_idx, double **values,
+                                       double **W_in, double **W_out);
+
+/* Largest |eigenvalue| estimate of a CSR matrix by the reference's power
+ * iteration (sparse.c:93-127: start from ones, max-norm, |d lambda| < 1e-6 or
+ * 1000 iterations), as used to rescale W to the requested spectral radius. */
+double spires_ext_spectral_radius(size_t n, const int64_t *row_ptr, const int32_t *col_idx,
+                                  const double *values);
+
+/* Run T macro steps over u [T x Din] and copy back any of: states [T x N]
+ * (V after each step), raster [T x N] (u8 spikes), outputs [T x Dout]. */
+spires_status spires_ext_run_record(spires_reservoir *r, const double *u, size_t T,
+                                    double *states, uint8_t *raster, double *outputs);
+
+/* Device-resident inputs: upload once, then run slices asynchronously on the
+ * handle's stream; records stay in HBM until read. */
+#define SPIRES_EXT_REC_STATE  1u
+#define SPIRES_EXT_REC_RASTER 2u
+#define SPIRES_EXT_REC_OUTPUT 4u
+spires_status spires_ext_upload_inputs(spires_reservoir *r, const double *u, size_t T);
+spires_status spires_ext_reserve_records(spires_reservoir *r, size_t rows, unsigned flags);
+spires_status spires_ext_run_resident(spires_reservoir *r, size_t t0, size_t T, unsigned flags);
+spires_status spires_ext_read_records(spires_reservoir *r, size_t row0, size_t rows,
+                                      double *states, uint8_t *raster, double *outputs);
+
+/* Ridge readout with a washout: reset, run u [T x Din], drop the first
+ * `washout` states, fit W_out on targets y [(T - washout) x Dout].
+ * info (optional, 2 doubles): [0] = 1 if the LU fallback was used,
+ * [1] = relative residual of the normal equations after refinement. */
+spires_status spires_ext_train_ridge_washout(spires_reservoir *r, const double *u,
+                                             const double *y, size_t T, size_t washout,
+                                             double lambda, double *info);
+
+/* Step arithmetic. */
+#define SPIRES_EXT_MODE_EXACT 0   /* FP64, bit-exact with the reference CPU path (default) */
+#define SPIRES_EXT_MODE_FAST  1   /* FP32 time-blocked GL history on matrix cores          */
+spires_status spires_ext_set_mode(spires_reservoir *r, int mode, int block_k);
+
+spires_status spires_ext_set_state(spires_reservoir *r, const double *V);
+spires_status spires_ext_sync(spires_reservoir *r);
+void         *spires_ext_stream(spires_reservoir *r);
+size_t        spires_ext_nnz(const spires_reservoir *r);
+spires_status spires_ext_get_csr(const spires_reservoir *r, int64_t *row_ptr,
+                                 int32_t *col_idx, double *values);
+spires_status spires_ext_get_input_weights(const spires_reservoir *r, double *W_in);
+const char   *spires_ext_last_error(void);
+
+/* AGILE (Levy exploration + SPSA refinement, the search behind
+ * spires_optimize) on a caller-supplied loss.  opts[11] = {mu, eta_min,
+ * eta_max, patience0, patience_decay, refine_step, refine_max_iters,
+ * refine_no_improve, spsa_eps, reflect_at_bounds, seed}.  Returns 0. */
+int spires_ext_agile_optimize(const double *theta0, const double *lo, const double *hi, int dim,
+                              double (*loss)(const double *theta, int dim, void *ctx), void *ctx,
+                              const double *opts, double *theta_best);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,368 @@
+"""spires_amd -- MI355X (gfx950) reservoir engine behind the spires C ABI.
+
+This module is a thin ctypes binding of ``libspires_amd.so`` (built from
+``spires_amd/csrc``): the drop-in ``include/spires.h`` API plus the
+extensions of ``include/spires_ext.h``.  There is no CPU fallback: if the
+HIP library is missing, ``lib()`` raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libspires_amd.so")
+
+NEURON = {"lif_discrete": 0, "lif_bio": 1, "flif_caputo": 2, "flif_gl": 3, "flif_diffusive": 4}
+SYNAPSE = {"simple": 0, "psc_homogeneous": 1, "psc_heterogeneous": 2, "fractional": 3,
+           "fractional_multiexp": 4}
+CONN = {"random": 0, "small_world": 1, "scale_free": 2}
+BACKEND = {"sparse": 0, "dense": 1}
+STATUS = {0: "SPIRES_OK", 1: "SPIRES_ERR_INVALID_ARG", 2: "SPIRES_ERR_ALLOC", 3: "SPIRES_ERR_INTERNAL"}
+
+REC_STATE, REC_RASTER, REC_OUTPUT = 1, 2, 4
+MODE_EXACT, MODE_FAST = 0, 1
+
+
+class SpiresError(RuntimeError):
+    def __init__(self, status, msg=""):
+        super().__init__(f"{STATUS.get(status, status)}: {msg}")
+        self.status = status
+
+
+class Config(C.Structure):
+    """spires_reservoir_config (include/spires.h)."""
+    _fields_ = [
+        ("num_neurons", C.c_size_t), ("num_inputs", C.c_size_t), ("num_outputs", C.c_size_t),
+        ("spectral_radius", C.c_double), ("ei_ratio", C.c_double), ("input_strength", C.c_double),
+        ("connectivity", C.c_double), ("dt", C.c_double),
+        ("connectivity_type", C.c_int), ("neuron_type", C.c_int),
+        ("neuron_params", C.POINTER(C.c_double)),
+        ("synapse_type", C.c_int), ("synapse_backend", C.c_int),
+        ("synapse_params", C.POINTER(C.c_double)),
+    ]
+
+
+class OptBudget(C.Structure):
+    _fields_ = [("data_fraction", C.c_double), ("num_seeds", C.c_int), ("time_limit_sec", C.c_double),
+                ("_reserved_i0", C.c_int), ("_reserved_d0", C.c_double)]
+
+
+class OptScore(C.Structure):
+    _fields_ = [("lambda_var", C.c_double), ("lambda_cost", C.c_double), ("metric", C.c_int),
+                ("_reserved_i0", C.c_int)]
+
+
+class OptResult(C.Structure):
+    _fields_ = [("best_config", Config), ("best_log10_ridge", C.c_double), ("best_score", C.c_double),
+                ("metric_mean", C.c_double), ("metric_std", C.c_double)]
+
+
+LOSS_FN = C.CFUNCTYPE(C.c_double, C.POINTER(C.c_double), C.c_int, C.c_void_p)
+
+_D = C.POINTER(C.c_double)
+_I64 = C.POINTER(C.c_int64)
+_I32 = C.POINTER(C.c_int32)
+_U8 = C.POINTER(C.c_uint8)
+_P = C.c_void_p
+_SZ = C.c_size_t
+
+# Every symbol of include/spires.h and include/spires_ext.h: name -> (restype, argtypes)
+SIGNATURES = {
+    "spires_reservoir_create": (C.c_int, [C.POINTER(Config), C.POINTER(_P)]),
+    "spires_reservoir_destroy": (None, [_P]),
+    "spires_reservoir_reset": (C.c_int, [_P]),
+    "spires_step": (C.c_int, [_P, _D]),
+    "spires_run": (_D, [_P, _D, _SZ]),
+    "spires_train_online": (C.c_int, [_P, _D, C.c_double]),
+    "spires_train_ridge": (C.c_int, [_P, _D, _D, _SZ, C.c_double]),
+    "spires_train_rls": (C.c_int, [_P, _D, _D, _SZ, C.c_double, C.c_double]),
+    "spires_coarse_grain": (C.c_int, [_P, C.c_double, C.POINTER(_P)]),
+    "spires_copy_reservoir_state": (_D, [_P]),
+    "spires_read_reservoir_state": (C.c_int, [_P, _D]),
+    "spires_read_spike_state": (C.c_int, [_P, _D]),
+    "spires_copy_weights": (_D, [_P]),
+    "spires_read_weights": (C.c_int, [_P, _D]),
+    "spires_compute_output": (C.c_int, [_P, _D]),
+    "spires_copy_readout": (_D, [_P]),
+    "spires_read_readout": (C.c_int, [_P, _D]),
+    "spires_num_neurons": (_SZ, [_P]),
+    "spires_num_inputs": (_SZ, [_P]),
+    "spires_num_outputs": (_SZ, [_P]),
+    "spires_optimize": (C.c_int, [C.POINTER(Config), C.POINTER(OptBudget), C.c_int, C.POINTER(OptScore),
+                                  C.POINTER(OptResult), _D, _D, _SZ]),
+    "spires_ext_create_csr": (C.c_int, [C.POINTER(Config), _SZ, _I64, _I32, _D, _D, _D, C.POINTER(_P)]),
+    "spires_ext_run_record": (C.c_int, [_P, _D, _SZ, _D, _U8, _D]),
+    "spires_ext_upload_inputs": (C.c_int, [_P, _D, _SZ]),
+    "spires_ext_reserve_records": (C.c_int, [_P, _SZ, C.c_uint]),
+    "spires_ext_run_resident": (C.c_int, [_P, _SZ, _SZ, C.c_uint]),
+    "spires_ext_read_records": (C.c_int, [_P, _SZ, _SZ, _D, _U8, _D]),
+    "spires_ext_train_ridge_washout": (C.c_int, [_P, _D, _D, _SZ, _SZ, C.c_double, _D]),
+    "spires_ext_set_mode": (C.c_int, [_P, C.c_int, C.c_int]),
+    "spires_ext_set_state": (C.c_int, [_P, _D]),
+    "spires_ext_sync": (C.c_int, [_P]),
+    "spires_ext_stream": (_P, [_P]),
+    "spires_ext_nnz": (_SZ, [_P]),
+    "spires_ext_get_csr": (C.c_int, [_P, _I64, _I32, _D]),
+    "spires_ext_get_input_weights": (C.c_int, [_P, _D]),
+    "spires_ext_last_error": (C.c_char_p, []),
+    "spires_ext_agile_optimize": (C.c_int, [_D, _D, _D, C.c_int, LOSS_FN, _P, _D, _D]),
+    "spires_ext_spectral_radius": (C.c_double, [_SZ, _I64, _I32, _D]),
+    "spires_ext_build_weights": (C.c_int, [C.POINTER(Config), C.POINTER(_SZ), C.POINTER(_I64), C.POINTER(_I32),
+                                           C.POINTER(_D), C.POINTER(_D), C.POINTER(_D)]),
+}
+
+_lib = None
+_libc = C.CDLL("libc.so.6")
+_libc.free.argtypes = [_P]
+
+
+def build(force: bool = True) -> None:
+    """Compile libspires_amd.so for gfx950 (hipcc cross-compiles without a GPU)."""
+    args = ["make", "-s", "-j8", "-C", HERE]
+    if force:
+        args.insert(2, "-B")
+    subprocess.run(args, check=True)
+
+
+def lib():
+    """Load the HIP library (raises if it was never built: there is no CPU path)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} missing: build it with spires_amd.build() "
+                               "(the engine has no CPU fallback)")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def srand(seed: int) -> None:
+    """glibc srand(): the reference's (and this library's) construction RNG."""
+    _libc.srand(C.c_uint(seed))
+
+
+def last_error() -> str:
+    m = lib().spires_ext_last_error()
+    return m.decode() if m else ""
+
+
+def _check(status):
+    if status != 0:
+        raise SpiresError(status, last_error())
+
+
+def _dp(a):
+    return None if a is None else a.ctypes.data_as(_D)
+
+
+def _f64(a):
+    return None if a is None else np.ascontiguousarray(a, dtype=np.float64)
+
+
+def make_config(n, din=1, dout=1, rho=0.9, ei=0.8, gain=1.0, conn=0.1, dt=1.0,
+                conn_type="random", neuron="flif_gl", params=(), synapse="simple",
+                backend="sparse", syn_params=None):
+    """(Config, owned arrays): the arrays must outlive any reservoir made from
+    the config, because the library retains the parameter pointers (as the
+    reference does)."""
+    np_arr = (C.c_double * max(1, len(params)))(*params)
+    sp_arr = None
+    if syn_params is not None:
+        sp_arr = (C.c_double * max(1, len(syn_params)))(*syn_params)
+    cfg = Config(n, din, dout, rho, ei, gain, conn, dt, CONN.get(conn_type, conn_type),
+                 NEURON.get(neuron, neuron), C.cast(np_arr, _D), SYNAPSE.get(synapse, synapse),
+                 BACKEND.get(backend, backend), C.cast(sp_arr, _D) if sp_arr is not None else None)
+    return cfg, np_arr, sp_arr
+
+
+def spectral_radius(rp, ci, val):
+    """Reference power-iteration estimate of rho(W) for a CSR matrix."""
+    rp = np.ascontiguousarray(rp, np.int64)
+    ci = np.ascontiguousarray(ci, np.int32)
+    val = _f64(val)
+    return lib().spires_ext_spectral_radius(len(rp) - 1, rp.ctypes.data_as(_I64), ci.ctypes.data_as(_I32),
+                                            _dp(val))
+
+
+def build_weights(cfg_tuple):
+    """(row_ptr, col_idx, values, W_in, W_out) that spires_reservoir_create
+    would build for this config (consumes glibc rand() the same way)."""
+    cfg = cfg_tuple[0]
+    nnz = _SZ()
+    rp, ci, val, wi, wo = _I64(), _I32(), _D(), _D(), _D()
+    _check(lib().spires_ext_build_weights(C.byref(cfg), C.byref(nnz), C.byref(rp), C.byref(ci), C.byref(val),
+                                          C.byref(wi), C.byref(wo)))
+    n, k = cfg.num_neurons, nnz.value
+    out = (np.ctypeslib.as_array(rp, shape=(n + 1,)).copy(),
+           np.ctypeslib.as_array(ci, shape=(max(k, 1),))[:k].copy(),
+           np.ctypeslib.as_array(val, shape=(max(k, 1),))[:k].copy(),
+           np.ctypeslib.as_array(wi, shape=(max(n * cfg.num_inputs, 1),))[:n * cfg.num_inputs].copy()
+           .reshape(n, cfg.num_inputs),
+           np.ctypeslib.as_array(wo, shape=(max(n * cfg.num_outputs, 1),))[:n * cfg.num_outputs].copy()
+           .reshape(cfg.num_outputs, n))
+    for p in (rp, ci, val, wi, wo):
+        _libc.free(C.cast(p, _P))
+    return out
+
+
+class Reservoir:
+    """Object view of a spires_reservoir handle (spires.h names)."""
+
+    def __init__(self, cfg_tuple, csr=None, W_in=None, W_out=None):
+        self._keep = cfg_tuple
+        cfg = cfg_tuple[0]
+        self.N, self.Din, self.Dout = cfg.num_neurons, cfg.num_inputs, cfg.num_outputs
+        h = _P()
+        L = lib()
+        if csr is None:
+            _check(L.spires_reservoir_create(C.byref(cfg), C.byref(h)))
+        else:
+            rp, ci, val = csr
+            self._rp = np.ascontiguousarray(rp, np.int64)
+            self._ci = np.ascontiguousarray(ci, np.int32)
+            self._val = _f64(val)
+            wi = _f64(W_in)
+            wo = _f64(W_out)
+            _check(L.spires_ext_create_csr(C.byref(cfg), len(self._val), self._rp.ctypes.data_as(_I64),
+                                           self._ci.ctypes.data_as(_I32), _dp(self._val), _dp(wi), _dp(wo),
+                                           C.byref(h)))
+        self.h = h
+
+    @classmethod
+    def _wrap(cls, h, keep, N, Din, Dout):
+        r = cls.__new__(cls)
+        r.h, r._keep, r.N, r.Din, r.Dout = h, keep, N, Din, Dout
+        return r
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().spires_reservoir_destroy(self.h)
+            self.h = None
+
+    __del__ = close
+
+    def reset(self):
+        _check(lib().spires_reservoir_reset(self.h))
+
+    def step(self, u=None):
+        u = _f64(u)
+        _check(lib().spires_step(self.h, _dp(u)))
+
+    def run(self, u):
+        u = _f64(u)
+        T = len(u) // max(1, self.Din) if u.ndim == 1 else u.shape[0]
+        p = lib().spires_run(self.h, _dp(u), T)
+        if not p:
+            raise SpiresError(3, last_error())
+        y = np.ctypeslib.as_array(p, shape=(T * self.Dout,)).copy().reshape(T, self.Dout)
+        _libc.free(p)
+        return y
+
+    def run_record(self, u, T=None, states=True, raster=True, outputs=False):
+        u = _f64(u)
+        if T is None:
+            T = u.shape[0] if u is not None else 0
+        X = np.empty((T, self.N)) if states else None
+        R = np.empty((T, self.N), np.uint8) if raster else None
+        Y = np.empty((T, self.Dout)) if outputs else None
+        _check(lib().spires_ext_run_record(self.h, _dp(u), T, _dp(X),
+                                           None if R is None else R.ctypes.data_as(_U8), _dp(Y)))
+        return X, R, Y
+
+    def upload_inputs(self, u):
+        u = _f64(u)
+        _check(lib().spires_ext_upload_inputs(self.h, _dp(u), u.shape[0]))
+
+    def reserve_records(self, rows, flags):
+        _check(lib().spires_ext_reserve_records(self.h, rows, flags))
+
+    def run_resident(self, t0, T, flags=0):
+        _check(lib().spires_ext_run_resident(self.h, t0, T, flags))
+
+    def read_records(self, row0, rows, states=False, raster=False, outputs=False):
+        X = np.empty((rows, self.N)) if states else None
+        R = np.empty((rows, self.N), np.uint8) if raster else None
+        Y = np.empty((rows, self.Dout)) if outputs else None
+        _check(lib().spires_ext_read_records(self.h, row0, rows, _dp(X),
+                                             None if R is None else R.ctypes.data_as(_U8), _dp(Y)))
+        return X, R, Y
+
+    def sync(self):
+        _check(lib().spires_ext_sync(self.h))
+
+    def stream(self):
+        return lib().spires_ext_stream(self.h)
+
+    def set_mode(self, mode, block_k=0):
+        _check(lib().spires_ext_set_mode(self.h, mode, block_k))
+
+    def train_ridge(self, u, y, lam):
+        u, y = _f64(u), _f64(y)
+        _check(lib().spires_train_ridge(self.h, _dp(u), _dp(y), u.shape[0], lam))
+
+    def train_ridge_washout(self, u, y, washout, lam):
+        u, y = _f64(u), _f64(y)
+        info = np.zeros(2)
+        _check(lib().spires_ext_train_ridge_washout(self.h, _dp(u), _dp(y), u.shape[0], washout, lam, _dp(info)))
+        return info
+
+    def train_rls(self, u, y, delta, lam):
+        u, y = _f64(u), _f64(y)
+        _check(lib().spires_train_rls(self.h, _dp(u), _dp(y), u.shape[0], delta, lam))
+
+    def train_online(self, target, lr):
+        _check(lib().spires_train_online(self.h, _dp(_f64(target)), lr))
+
+    def coarse_grain(self, thr):
+        h = _P()
+        _check(lib().spires_coarse_grain(self.h, thr, C.byref(h)))
+        return Reservoir._wrap(h, self._keep, lib().spires_num_neurons(h), self.Din, self.Dout)
+
+    def state(self):
+        v = np.empty(self.N)
+        _check(lib().spires_read_reservoir_state(self.h, _dp(v)))
+        return v
+
+    def set_state(self, V):
+        _check(lib().spires_ext_set_state(self.h, _dp(_f64(V))))
+
+    def spikes(self):
+        s = np.empty(self.N)
+        _check(lib().spires_read_spike_state(self.h, _dp(s)))
+        return s
+
+    def output(self):
+        y = np.empty(self.Dout)
+        _check(lib().spires_compute_output(self.h, _dp(y)))
+        return y
+
+    def weights(self):
+        W = np.empty((self.N, self.N))
+        _check(lib().spires_read_weights(self.h, _dp(W)))
+        return W
+
+    def csr(self):
+        nnz = lib().spires_ext_nnz(self.h)
+        rp = np.empty(self.N + 1, np.int64)
+        ci = np.empty(max(nnz, 1), np.int32)
+        val = np.empty(max(nnz, 1))
+        _check(lib().spires_ext_get_csr(self.h, rp.ctypes.data_as(_I64), ci.ctypes.data_as(_I32), _dp(val)))
+        return rp, ci[:nnz], val[:nnz]
+
+    def W_in(self):
+        W = np.empty((self.N, self.Din))
+        _check(lib().spires_ext_get_input_weights(self.h, _dp(W)))
+        return W
+
+    def W_out(self):
+        W = np.empty((self.Dout, self.N))
+        _check(lib().spires_read_readout(self.h, _dp(W)))
+        return W

@@ -1,0 +1,110 @@
+// Internal definitions shared by the device-layer translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <cstddef>
+#include <string>
+
+#include "spires_dev.h"
+
+namespace sdev {
+
+enum { NT_LIF_DISCRETE = 0, NT_LIF_BIO, NT_FLIF_CAPUTO, NT_FLIF_GL, NT_FLIF_DIFFUSIVE };
+enum { ST_SIMPLE = 0, ST_PSC_HOM, ST_PSC_HET, ST_FRACTIONAL, ST_MULTIEXP };
+enum { BK_SPARSE = 0, BK_DENSE = 1 };
+
+void set_error(const std::string& msg);
+int hip_fail(hipError_t e, const char* what, int line);
+
+#define SDEV_CK(x)                                                        \
+    do {                                                                  \
+        hipError_t e_ = (x);                                              \
+        if (e_ != hipSuccess) return ::sdev::hip_fail(e_, #x, __LINE__);  \
+    } while (0)
+
+template <typename T>
+int dalloc(T** p, size_t count) {
+    *p = nullptr;
+    if (count == 0) count = 1;
+    hipError_t e = hipMalloc((void**)p, count * sizeof(T));
+    if (e != hipSuccess) {
+        *p = nullptr;
+        (void)hipGetLastError();
+        set_error(std::string("hipMalloc failed: ") + hipGetErrorString(e));
+        return SPIRES_DEV_EALLOC;
+    }
+    return 0;
+}
+template <typename T>
+void dfree(T*& p) {
+    if (p) (void)hipFree((void*)p);
+    p = nullptr;
+}
+
+// Input vector passed by value for host-driven steps (no copy needed).
+constexpr int kMaxInlineInputs = 32;
+struct InlineU { double v[kMaxInlineInputs]; };
+
+// Arguments of the per-micro-step kernel (passed by value).
+struct StepArgs {
+    int n, din, L, m;           // m: Caputo register length (L-2)
+    int backend, stype;
+    long long st;               // micro-step index since reset
+    int head, lim;              // GL ring head and lag limit for this step
+    double gain, dt;
+    double np[16];
+    const double* W_in;
+    const double* u;            // device input row (nullptr -> inline)
+    InlineU uin;
+    int has_u;
+    const int64_t* rp; const int32_t* col; const double* val; const double* WT;
+    const uint8_t* S_prev; uint8_t* S_cur;
+    const double* tr_prev; double* tr_cur; double syn_decay;
+    const double* edge_decay; double* edge_tr;
+    const double* eff;
+    double* V; double* tprev; double* ring; const double* coef;
+    int* chead; long long* until; unsigned long long* rng;
+    double* rec_V; uint8_t* rec_S;
+};
+
+}  // namespace sdev
+
+struct spires_dev {
+    spires_dev_desc d;          // scalars (pointer members not retained)
+    hipStream_t stream = nullptr;
+    int n = 0, din = 0, dout = 0, micro = 1;
+    int ntype = 0, stype = 0, backend = 0;
+    int L = 0, m = 0;
+    // weights
+    double* W_in = nullptr; double* W_out = nullptr;
+    int64_t* rp = nullptr; int32_t* col = nullptr; double* val = nullptr; int64_t nnz = 0;
+    double* WT = nullptr;       // dense backend, transposed (column j contiguous over rows)
+    // neuron state
+    double* coef = nullptr; int ncoef = 0;
+    double* V = nullptr; uint8_t* S[2] = {nullptr, nullptr}; int cur = 0;
+    double* tprev = nullptr; double* ring = nullptr; size_t ring_elems = 0;
+    int* chead = nullptr; long long* until = nullptr; unsigned long long* rng = nullptr;
+    // synapse state (survives reset)
+    double* tr[2] = {nullptr, nullptr}; int trcur = 0;
+    double* edge_decay = nullptr; double* edge_tr = nullptr;
+    double* frac_coef = nullptr; double* frac_hist = nullptr; double* frac_I = nullptr;
+    int frac_L = 0; long long frac_step = 0; double frac_da = 0, frac_tau = 1;
+    int me_n = 0; double me_decay[64]; double me_c[64]; double* me_tr = nullptr; double* me_comb = nullptr;
+    // inputs and records
+    double* u_series = nullptr; size_t u_T = 0;
+    double* rec_state = nullptr; uint8_t* rec_raster = nullptr; double* rec_out = nullptr;
+    size_t rec_rows = 0, rec_cap_s = 0, rec_cap_r = 0, rec_cap_o = 0;
+    double* ybuf = nullptr;
+    double* ustep = nullptr;    // device row for host steps with din > inline limit
+    long long step = 0;         // micro-steps since reset
+    size_t msteps = 0;          // macro steps since reset
+    int mode = SPIRES_DEV_MODE_EXACT, block_k = 0;
+    void* readout_ws = nullptr; // rocBLAS/rocSOLVER handles (readout unit)
+};
+
+namespace sdev {
+int launch_step(spires_dev* g, const double* u_dev, const double* u_host, double* rec_V,
+                uint8_t* rec_S);
+int launch_output(spires_dev* g, double* y_dev);
+void readout_release(spires_dev* g);
+}  // namespace sdev

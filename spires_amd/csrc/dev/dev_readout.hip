@@ -1,0 +1,300 @@
+// Readout training on the device.
+//
+// Ridge (reservoir.c:573-679): the recorded state matrix X [T x N] stays in
+// HBM.  A = X'X (+ lambda I) and B = X'Y are formed on the GPU, then one FP64
+// Cholesky factorisation solves all Dout right-hand sides at once (the
+// reference refactorises A with LU once per output), followed by FP64
+// iterative refinement against the unfactored A.  A matrix that is not
+// positive definite falls back to LU with partial pivoting.
+//
+// RLS (reservoir.c:681-767) and the delta rule (reservoir.c:244-280) run as
+// fused FP64 kernels on the resident P / W_out.
+#include "dev_internal.hpp"
+
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
+
+#include <cmath>
+#include <vector>
+
+namespace sdev {
+
+struct ReadoutWS {
+    rocblas_handle h = nullptr;
+};
+
+static rocblas_handle get_handle(spires_dev* g) {
+    ReadoutWS* ws = (ReadoutWS*)g->readout_ws;
+    if (!ws) {
+        ws = new ReadoutWS();
+        if (rocblas_create_handle(&ws->h) != rocblas_status_success) { delete ws; return nullptr; }
+        g->readout_ws = ws;
+    }
+    rocblas_set_stream(ws->h, g->stream);
+    rocblas_set_pointer_mode(ws->h, rocblas_pointer_mode_host);
+    return ws->h;
+}
+
+void readout_release(spires_dev* g) {
+    ReadoutWS* ws = (ReadoutWS*)g->readout_ws;
+    if (ws) {
+        if (ws->h) rocblas_destroy_handle(ws->h);
+        delete ws;
+    }
+    g->readout_ws = nullptr;
+}
+
+#define SDEV_RB(x)                                                                  \
+    do {                                                                            \
+        rocblas_status s_ = (x);                                                    \
+        if (s_ != rocblas_status_success) {                                         \
+            set_error(std::string(#x) + " failed: " + rocblas_status_to_string(s_)); \
+            return SPIRES_DEV_EINTERNAL;                                            \
+        }                                                                           \
+    } while (0)
+
+// A[i,i] += lambda; mirror the lower triangle into the upper one.
+__global__ void k_ridge_diag_sym(int n, double* A, double lambda) {
+    const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (size_t)n * n) return;
+    const int c = (int)(idx / n), r = (int)(idx % n);   // column-major (r, c)
+    if (r == c) A[idx] = A[idx] + lambda;
+    else if (r < c) A[idx] = A[(size_t)r * n + c];      // upper <- lower (c, r)
+}
+
+__global__ void k_axpy(size_t n, const double* x, double* y) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) y[i] = y[i] + x[i];
+}
+
+// ---- RLS, one step: P x, gain, error, rank-1 updates (reservoir.c:721-760)
+__global__ void __launch_bounds__(256) k_rls_px(int n, const double* __restrict__ P,
+                                                const double* __restrict__ x, double* px) {
+    __shared__ double red[256];
+    const int row = blockIdx.x;
+    double s = 0.0;
+    for (int j = threadIdx.x; j < n; j += 256) s += P[(size_t)row * n + j] * x[j];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) px[row] = red[0];
+}
+
+// single block: den = lambda + x.Px; k = Px/den; e = y - W x
+__global__ void __launch_bounds__(256) k_rls_gain(int n, int dout, double lambda, const double* x,
+                                                  double* px, const double* W, const double* y,
+                                                  double* e, double* den_out) {
+    __shared__ double red[256];
+    __shared__ double den_s;
+    double s = 0.0;
+    for (int j = threadIdx.x; j < n; j += 256) s += x[j] * px[j];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) { den_s = lambda + red[0]; *den_out = den_s; }
+    __syncthreads();
+    const double sc = 1.0 / den_s;
+    for (int o = 0; o < dout; ++o) {
+        double t = 0.0;
+        for (int j = threadIdx.x; j < n; j += 256) t += W[(size_t)o * n + j] * x[j];
+        red[threadIdx.x] = t;
+        __syncthreads();
+        for (int w = 128; w > 0; w >>= 1) {
+            if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) e[o] = y[o] - red[0];
+        __syncthreads();
+    }
+    for (int j = threadIdx.x; j < n; j += 256) px[j] = sc * px[j];
+}
+
+__global__ void k_rls_update(int n, int dout, double lambda, const double* k, const double* e,
+                             const double* den, double* P, double* W) {
+    const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t nn = (size_t)n * n;
+    if (idx < nn) {
+        const int i = (int)(idx / n), j = (int)(idx % n);
+        double p = P[idx] + (-(*den) * k[i]) * k[j];
+        if (lambda != 1.0) p = (1.0 / lambda) * p;
+        P[idx] = p;
+    } else if (idx < nn + (size_t)n * dout) {
+        const size_t w = idx - nn;
+        const int o = (int)(w / n), j = (int)(w % n);
+        W[w] = W[w] + (1.0 * e[o]) * k[j];
+    }
+}
+
+__global__ void k_eye(int n, double* P, double v) {
+    const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx < (size_t)n * n) P[idx] = (idx / n == idx % n) ? v : 0.0;
+}
+
+// delta rule: W[o,:] += (lr * (t_o - y_o)) * V
+__global__ void k_online(int n, int dout, double lr, const double* t, const double* y,
+                         const double* V, double* W) {
+    const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (size_t)n * dout) return;
+    const int o = (int)(idx / n), j = (int)(idx % n);
+    W[idx] = W[idx] + (lr * (t[o] - y[o])) * V[j];
+}
+
+static inline unsigned nb(size_t n, unsigned b) { return (unsigned)((n + b - 1) / b); }
+
+}  // namespace sdev
+
+using namespace sdev;
+
+extern "C" {
+
+int spires_dev_ridge(spires_dev* g, size_t washout, const double* Y, double lambda, double* info_out) {
+    hipError_t herr = hipSuccess;
+    auto H = [&](hipError_t e) { if (e != hipSuccess && herr == hipSuccess) herr = e; };
+    if (!g || !Y) return SPIRES_DEV_EINVAL;
+    if (!g->rec_state || washout >= g->rec_rows) { set_error("no recorded states for ridge"); return SPIRES_DEV_EINVAL; }
+    const int n = g->n, D = g->dout;
+    const int T = (int)(g->rec_rows - washout);
+    if (D == 0) return 0;
+    rocblas_handle h = get_handle(g);
+    if (!h) { set_error("rocblas_create_handle failed"); return SPIRES_DEV_EINTERNAL; }
+    double *A = nullptr, *A0 = nullptr, *B = nullptr, *X = nullptr, *R = nullptr, *Yd = nullptr;
+    rocblas_int* info = nullptr;
+    rocblas_int* ipiv = nullptr;
+    int rc = 0;
+    auto cleanup = [&]() { dfree(A); dfree(A0); dfree(B); dfree(X); dfree(R); dfree(Yd); dfree(info); dfree(ipiv); };
+    const size_t nn = (size_t)n * n, nd = (size_t)n * D;
+    if ((rc = dalloc(&A, nn)) || (rc = dalloc(&A0, nn)) || (rc = dalloc(&B, nd)) || (rc = dalloc(&X, nd)) ||
+        (rc = dalloc(&R, nd)) || (rc = dalloc(&Yd, (size_t)T * D)) || (rc = dalloc(&info, 1))) {
+        cleanup();
+        return rc;
+    }
+    const double* Xs = g->rec_state + washout * (size_t)n;   // column-major n x T (ld n) == X'
+    const double one = 1.0, zero = 0.0, mone = -1.0;
+    H(hipMemcpyAsync(Yd, Y, (size_t)T * D * 8, hipMemcpyHostToDevice, g->stream));
+    // A = X'X (lower), B = X'Y
+    rocblas_status s1 = rocblas_dsyrk(h, rocblas_fill_lower, rocblas_operation_none, n, T, &one, Xs, n, &zero, A, n);
+    rocblas_status s2 = rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_transpose, n, D, T, &one, Xs, n,
+                                      Yd, D, &zero, B, n);
+    if (s1 != rocblas_status_success || s2 != rocblas_status_success) {
+        cleanup();
+        set_error("Gram/cross products failed");
+        return SPIRES_DEV_EINTERNAL;
+    }
+    hipLaunchKernelGGL(k_ridge_diag_sym, dim3(nb(nn, 256)), dim3(256), 0, g->stream, n, A, lambda);
+    H(hipMemcpyAsync(A0, A, nn * 8, hipMemcpyDeviceToDevice, g->stream));
+    H(hipMemcpyAsync(X, B, nd * 8, hipMemcpyDeviceToDevice, g->stream));
+    rocblas_status s3 = rocsolver_dpotrf(h, rocblas_fill_lower, n, A, n, info);
+    rocblas_int hinfo = 0;
+    H(hipMemcpyAsync(&hinfo, info, sizeof(hinfo), hipMemcpyDeviceToHost, g->stream));
+    H(hipStreamSynchronize(g->stream));
+    bool use_lu = (s3 != rocblas_status_success) || hinfo != 0;
+    if (use_lu) {
+        if ((rc = dalloc(&ipiv, (size_t)n))) { cleanup(); return rc; }
+        H(hipMemcpyAsync(A, A0, nn * 8, hipMemcpyDeviceToDevice, g->stream));
+        rocblas_status s4 = rocsolver_dgetrf(h, n, n, A, n, ipiv, info);
+        rocblas_int i2 = 0;
+        H(hipMemcpyAsync(&i2, info, sizeof(i2), hipMemcpyDeviceToHost, g->stream));
+        H(hipStreamSynchronize(g->stream));
+        if (s4 != rocblas_status_success || i2 != 0) {
+            cleanup();
+            set_error("ridge system is singular");
+            if (info_out) info_out[0] = (double)i2;
+            return SPIRES_DEV_EINTERNAL;   // W_out left unchanged, as the reference does
+        }
+        rocsolver_dgetrs(h, rocblas_operation_none, n, D, A, n, ipiv, X, n);
+    } else {
+        rocsolver_dpotrs(h, rocblas_fill_lower, n, D, A, n, X, n);
+    }
+    // iterative refinement: R = B - A0 X ; solve ; X += dX
+    double rel = 0.0;
+    for (int it = 0; it < 2; ++it) {
+        H(hipMemcpyAsync(R, B, nd * 8, hipMemcpyDeviceToDevice, g->stream));
+        rocblas_dsymm(h, rocblas_side_left, rocblas_fill_lower, n, D, &mone, A0, n, X, n, &one, R, n);
+        if (use_lu) rocsolver_dgetrs(h, rocblas_operation_none, n, D, A, n, ipiv, R, n);
+        else rocsolver_dpotrs(h, rocblas_fill_lower, n, D, A, n, R, n);
+        hipLaunchKernelGGL(k_axpy, dim3(nb(nd, 256)), dim3(256), 0, g->stream, nd, R, X);
+    }
+    if (info_out) {
+        // final relative residual max_o ||B - A0 X|| / ||B||
+        H(hipMemcpyAsync(R, B, nd * 8, hipMemcpyDeviceToDevice, g->stream));
+        rocblas_dsymm(h, rocblas_side_left, rocblas_fill_lower, n, D, &mone, A0, n, X, n, &one, R, n);
+        std::vector<double> hr(nd), hb(nd);
+        H(hipMemcpyAsync(hr.data(), R, nd * 8, hipMemcpyDeviceToHost, g->stream));
+        H(hipMemcpyAsync(hb.data(), B, nd * 8, hipMemcpyDeviceToHost, g->stream));
+        H(hipStreamSynchronize(g->stream));
+        for (int o = 0; o < D; ++o) {
+            double a = 0, b = 0;
+            for (int i = 0; i < n; ++i) { a += hr[(size_t)o * n + i] * hr[(size_t)o * n + i]; b += hb[(size_t)o * n + i] * hb[(size_t)o * n + i]; }
+            if (b > 0) rel = fmax(rel, sqrt(a / b));
+        }
+        info_out[0] = use_lu ? 1.0 : 0.0;
+        info_out[1] = rel;
+    }
+    // X is column-major n x D == row-major W_out [D x n]
+    H(hipMemcpyAsync(g->W_out, X, nd * 8, hipMemcpyDeviceToDevice, g->stream));
+    H(hipStreamSynchronize(g->stream));
+    cleanup();
+    if (herr != hipSuccess) return hip_fail(herr, "ridge", __LINE__);
+    return 0;
+}
+
+int spires_dev_rls(spires_dev* g, const double* Y, size_t T, double delta, double lambda) {
+    hipError_t herr = hipSuccess;
+    auto H = [&](hipError_t e) { if (e != hipSuccess && herr == hipSuccess) herr = e; };
+    if (!g || (!Y && T)) return SPIRES_DEV_EINVAL;
+    if (T > g->u_T && g->din > 0) { set_error("rls beyond uploaded inputs"); return SPIRES_DEV_EINVAL; }
+    const int n = g->n, D = g->dout;
+    const size_t nn = (size_t)n * n;
+    double *P = nullptr, *k = nullptr, *e = nullptr, *den = nullptr, *Yd = nullptr;
+    int rc = 0;
+    auto cleanup = [&]() { dfree(P); dfree(k); dfree(e); dfree(den); dfree(Yd); };
+    if ((rc = dalloc(&P, nn)) || (rc = dalloc(&k, (size_t)n)) || (rc = dalloc(&e, (size_t)D + 1)) ||
+        (rc = dalloc(&den, 1)) || (rc = dalloc(&Yd, T * (size_t)D + 1))) {
+        cleanup();
+        return rc;
+    }
+    H(hipMemcpyAsync(Yd, Y, T * (size_t)D * 8, hipMemcpyHostToDevice, g->stream));
+    hipLaunchKernelGGL(k_eye, dim3(nb(nn, 256)), dim3(256), 0, g->stream, n, P, 1.0 / delta);
+    H(hipMemsetAsync(g->W_out, 0, (size_t)n * D * 8, g->stream));
+    for (size_t t = 0; t < T; ++t) {
+        const double* u = g->din > 0 ? g->u_series + t * g->din : nullptr;
+        if ((rc = launch_step(g, u, nullptr, nullptr, nullptr))) { cleanup(); return rc; }
+        hipLaunchKernelGGL(k_rls_px, dim3(n), dim3(256), 0, g->stream, n, P, g->V, k);
+        hipLaunchKernelGGL(k_rls_gain, dim3(1), dim3(256), 0, g->stream, n, D, lambda, g->V, k, g->W_out,
+                           Yd + t * D, e, den);
+        const size_t tot = nn + (size_t)n * D;
+        hipLaunchKernelGGL(k_rls_update, dim3(nb(tot, 256)), dim3(256), 0, g->stream, n, D, lambda, k, e, den,
+                           P, g->W_out);
+    }
+    H(hipStreamSynchronize(g->stream));
+    cleanup();
+    if (herr != hipSuccess) return hip_fail(herr, "rls", __LINE__);
+    return 0;
+}
+
+int spires_dev_online(spires_dev* g, const double* target, double lr) {
+    hipError_t herr = hipSuccess;
+    auto H = [&](hipError_t e) { if (e != hipSuccess && herr == hipSuccess) herr = e; };
+    if (!g || !target) return SPIRES_DEV_EINVAL;
+    const int n = g->n, D = g->dout;
+    if (D == 0) return 0;
+    double* td = nullptr;
+    int rc = dalloc(&td, (size_t)D);
+    if (rc) return rc;
+    H(hipMemcpyAsync(td, target, (size_t)D * 8, hipMemcpyHostToDevice, g->stream));
+    if ((rc = launch_output(g, g->ybuf))) { dfree(td); return rc; }
+    hipLaunchKernelGGL(k_online, dim3(nb((size_t)n * D, 256)), dim3(256), 0, g->stream, n, D, lr, td, g->ybuf,
+                       g->V, g->W_out);
+    H(hipStreamSynchronize(g->stream));
+    dfree(td);
+    if (herr != hipSuccess) return hip_fail(herr, "online", __LINE__);
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,91 @@
+"""Synthetic workloads of the benchmark configurations.
+
+Each builder returns plain numpy arrays (recurrent CSR, W_in, input series)
+plus the config tuple, so the same workload can be handed to the GPU engine
+and, in tests and the CPU baseline, to the oracle.
+
+Topologies come from the engine's own generator (the reference's
+init_weights, reservoir.c:345-545, with glibc rand()); weight values are then
+replaced by N(0,1) draws from a seeded numpy generator and rescaled to the
+requested spectral radius with the reference's power-iteration rule
+(sparse.c:93-127), because the reference itself can only draw +/-U[0,1]
+weights.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from . import make_config, build_weights, spectral_radius, srand
+
+
+# FLIF-GL parameter layout: [V_th, V_reset, V_rest, tau_m, alpha, T_mem, bias, t_ref]
+def flif_gl_params(alpha=0.8, T_mem=512.0, bias=0.05, t_ref=2.0, tau_m=20.0, V_th=1.0):
+    return [V_th, 0.0, 0.0, tau_m, alpha, float(T_mem), bias, t_ref]
+
+
+def gaussian_reservoir(n, topology, conn, rho, params, seed, neuron="flif_gl", din=1, dout=1, gain=1.0,
+                       dt=1.0, w_in_scale=1.0):
+    """Topology from the reference generator, N(0,1) weights rescaled to rho,
+    W_in ~ U[-w_in_scale, w_in_scale].  Returns (cfg_tuple, (rp, ci, val), W_in)."""
+    srand(seed)
+    rp, ci, _, _, _ = build_weights(make_config(n, din=din, dout=dout, conn=conn, conn_type=topology,
+                                                neuron=neuron, params=params, dt=dt))
+    rng = np.random.default_rng(seed)
+    val = rng.standard_normal(len(ci))
+    lam = spectral_radius(rp, ci, val)
+    if lam > 1e-9:
+        val = val * (rho / lam)
+    W_in = rng.uniform(-w_in_scale, w_in_scale, (n, din))
+    cfg = make_config(n, din=din, dout=dout, rho=rho, gain=gain, conn=conn, conn_type=topology, neuron=neuron,
+                      params=params, dt=dt)
+    return cfg, (rp, ci, val), W_in
+
+
+def config1(n=4096, T=20000, T_mem=512, seed=1):
+    """FP64 exact mode, single reservoir: FLIF-GL N=4096, L=512, Watts-Strogatz
+    k=20 p=0.1, N(0,1) weights at rho=0.95, gain 2.0, input U[0,0.5],
+    delayed-recall targets u[n-d], d=1..100."""
+    params = flif_gl_params(alpha=0.8, T_mem=T_mem, bias=0.05, t_ref=2.0)
+    cfg, csr, W_in = gaussian_reservoir(n, "small_world", 20.0 / (n - 1), 0.95, params, seed, din=1,
+                                        dout=100, gain=2.0)
+    u = np.random.default_rng(seed + 1000).uniform(0.0, 0.5, (T, 1))
+    return cfg, csr, W_in, u
+
+
+def delayed_recall_targets(u, delays):
+    """y[t, j] = u[t - d_j] (zero before the series starts)."""
+    T = u.shape[0]
+    y = np.zeros((T, len(delays)))
+    for j, d in enumerate(delays):
+        y[d:, j] = u[:T - d, 0]
+    return y
+
+
+def mackey_glass(T, seed=0, tau=17.0, beta=0.2, gamma=0.1, n=10, dt=0.1, every=1.0, washout=1000):
+    """Mackey-Glass dx/dt = beta x(t-tau)/(1+x(t-tau)^n) - gamma x, RK4 with
+    step dt; the delayed term at the half step is the mean of its grid
+    neighbours.  x0 = 1.2 + U[-0.1, 0.1] from the seed; history = x0.
+    Sampled every `every` time units after a washout of `washout` samples."""
+    rng = np.random.default_rng(seed)
+    x0 = 1.2 + rng.uniform(-0.1, 0.1)
+    lag = int(round(tau / dt))
+    sub = int(round(every / dt))
+    total = (T + washout) * sub + 1
+    x = np.empty(total + lag)
+    x[:lag + 1] = x0
+
+    def f(xt, xd):
+        return beta * xd / (1.0 + xd ** n) - gamma * xt
+
+    for k in range(lag, lag + total - 1):
+        xd0 = x[k - lag]
+        xd1 = x[k - lag + 1]
+        xdh = 0.5 * (xd0 + xd1)
+        xt = x[k]
+        k1 = f(xt, xd0)
+        k2 = f(xt + 0.5 * dt * k1, xdh)
+        k3 = f(xt + 0.5 * dt * k2, xdh)
+        k4 = f(xt + dt * k3, xd1)
+        x[k + 1] = xt + dt / 6.0 * (k1 + 2 * k2 + 2 * k3 + k4)
+    series = x[lag::sub][washout:washout + T]
+    return series.copy()
