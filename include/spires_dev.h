@@ -109,6 +109,9 @@ int  spires_dev_rls(spires_dev *g, const double *Y, size_t T, double delta, doub
 int  spires_dev_online(spires_dev *g, const double *target, double lr);
 
 int  spires_dev_set_mode(spires_dev *g, int mode, int block_k);
+/* Kernel path of exact-mode runs: 0 automatic (persistent kernel when the
+ * model supports it), 1 one launch per micro-step, 2 persistent only. */
+int  spires_dev_set_path(spires_dev *g, int path);
 int  spires_dev_sync(spires_dev *g);
 void *spires_dev_stream(spires_dev *g);
 size_t spires_dev_steps(const spires_dev *g);

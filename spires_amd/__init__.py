@@ -25,6 +25,7 @@ STATUS = {0: "SPIRES_OK", 1: "SPIRES_ERR_INVALID_ARG", 2: "SPIRES_ERR_ALLOC", 3:
 
 REC_STATE, REC_RASTER, REC_OUTPUT = 1, 2, 4
 MODE_EXACT, MODE_FAST = 0, 1
+PATH_AUTO, PATH_PER_STEP, PATH_PERSISTENT = 0, 1, 2
 
 
 class SpiresError(RuntimeError):
@@ -102,6 +103,7 @@ SIGNATURES = {
     "spires_ext_read_records": (C.c_int, [_P, _SZ, _SZ, _D, _U8, _D]),
     "spires_ext_train_ridge_washout": (C.c_int, [_P, _D, _D, _SZ, _SZ, C.c_double, _D]),
     "spires_ext_set_mode": (C.c_int, [_P, C.c_int, C.c_int]),
+    "spires_ext_set_kernel_path": (C.c_int, [_P, C.c_int]),
     "spires_ext_set_state": (C.c_int, [_P, _D]),
     "spires_ext_sync": (C.c_int, [_P]),
     "spires_ext_stream": (_P, [_P]),
@@ -303,6 +305,10 @@ class Reservoir:
 
     def set_mode(self, mode, block_k=0):
         _check(lib().spires_ext_set_mode(self.h, mode, block_k))
+
+    def set_kernel_path(self, path):
+        """0 automatic, 1 one launch per micro-step, 2 persistent kernel only."""
+        _check(lib().spires_ext_set_kernel_path(self.h, path))
 
     def train_ridge(self, u, y, lam):
         u, y = _f64(u), _f64(y)

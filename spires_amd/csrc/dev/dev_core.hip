@@ -369,6 +369,7 @@ static int init_neurons(spires_dev* g, const spires_dev_desc* d, bool full) {
         SDEV_CK(hipStreamSynchronize(g->stream));
         dfree(rng0);
     }
+    if (g->xch) SDEV_CK(hipMemsetAsync(g->xch, 0, g->xch_words * 8, g->stream));
     g->cur = 0;
     g->step = 0;
     g->msteps = 0;
@@ -497,6 +498,7 @@ void spires_dev_destroy(spires_dev* g) {
     dfree(g->edge_decay); dfree(g->edge_tr); dfree(g->frac_coef); dfree(g->frac_hist);
     dfree(g->frac_I); dfree(g->me_tr); dfree(g->me_comb); dfree(g->u_series);
     dfree(g->rec_state); dfree(g->rec_raster); dfree(g->rec_out); dfree(g->ybuf); dfree(g->ustep);
+    dfree(g->xch); dfree(g->abort_flag);
     if (g->stream) (void)hipStreamDestroy(g->stream);
     delete g;
 }
@@ -560,7 +562,18 @@ int spires_dev_create(const spires_dev_desc* d, spires_dev** out) {
         SDEV_CK(hipStreamSynchronize(g->stream));
     }
     // synapse state
-    if (g->stype == ST_PSC_HOM) {
+    if (g->stype == ST_PSC_HOM && g->backend == BK_SPARSE) {
+        // per-edge copies of the presynaptic trace: every copy of neuron j's trace sees
+        // the same operations (tr*d + (1-d)*s_j from the same start), so each equals the
+        // reference's per-neuron trace bit for bit, and the row pull needs only spike bits
+        g->stype = ST_PSC_HET;
+        const size_t m = (size_t)d->nnz;
+        std::vector<double> dec(m ? m : 1, d->syn_decay);
+        if ((rc = dalloc(&g->edge_decay, m)) || (rc = dalloc(&g->edge_tr, m))) return fail(rc);
+        if ((rc = up(g->edge_decay, dec.data(), m * 8))) return fail(rc);
+        SDEV_CK(hipMemsetAsync(g->edge_tr, 0, m * 8 + 8, g->stream));
+        SDEV_CK(hipStreamSynchronize(g->stream));
+    } else if (g->stype == ST_PSC_HOM) {
         if ((rc = dalloc(&g->tr[0], n)) || (rc = dalloc(&g->tr[1], n))) return fail(rc);
         SDEV_CK(hipMemsetAsync(g->tr[0], 0, n * 8, g->stream));
         SDEV_CK(hipMemsetAsync(g->tr[1], 0, n * 8, g->stream));
@@ -626,8 +639,20 @@ int spires_dev_upload_inputs(spires_dev* g, const double* u, size_t T) {
 int spires_dev_run(spires_dev* g, size_t t0, size_t T, unsigned flags) {
     if (!g) return SPIRES_DEV_EINVAL;
     if (t0 + T > g->u_T && g->din > 0) { set_error("run beyond uploaded inputs"); return SPIRES_DEV_EINVAL; }
-    int rc = ensure_records(g, T, flags);
+    const bool persist = g->mode == SPIRES_DEV_MODE_EXACT && g->path != 1 && T > 0 && persist_supported(g);
+    if (g->path == 2 && !persist) { set_error("persistent kernel cannot run this reservoir"); return SPIRES_DEV_EINVAL; }
+    unsigned need = flags;
+    if (persist && (flags & SPIRES_DEV_REC_OUTPUT)) need |= SPIRES_DEV_REC_STATE;   // outputs from the state record
+    int rc = ensure_records(g, T, need);
     if (rc) return rc;
+    if (persist) {
+        const double* u = g->din > 0 ? g->u_series : nullptr;
+        double* rv = (need & SPIRES_DEV_REC_STATE) ? g->rec_state : nullptr;
+        uint8_t* rs = (flags & SPIRES_DEV_REC_RASTER) ? g->rec_raster : nullptr;
+        if ((rc = launch_persist(g, u, (long long)t0, (long long)T, rv, rs))) return rc;
+        if (flags & SPIRES_DEV_REC_OUTPUT) return outputs_from_states(g, g->rec_state, T, g->rec_out);
+        return 0;
+    }
     for (size_t t = 0; t < T; ++t) {
         const double* u = g->din > 0 ? g->u_series + (t0 + t) * g->din : nullptr;
         double* rv = (flags & SPIRES_DEV_REC_STATE) ? g->rec_state + t * g->n : nullptr;
@@ -710,6 +735,12 @@ int spires_dev_set_mode(spires_dev* g, int mode, int block_k) {
     if (!g || (mode != SPIRES_DEV_MODE_EXACT && mode != SPIRES_DEV_MODE_FAST)) return SPIRES_DEV_EINVAL;
     g->mode = mode;
     g->block_k = block_k;
+    return 0;
+}
+
+int spires_dev_set_path(spires_dev* g, int path) {
+    if (!g || path < 0 || path > 2) return SPIRES_DEV_EINVAL;
+    g->path = path;
     return 0;
 }
 

@@ -100,6 +100,9 @@ struct spires_dev {
     size_t msteps = 0;          // macro steps since reset
     int mode = SPIRES_DEV_MODE_EXACT, block_k = 0;
     void* readout_ws = nullptr; // rocBLAS/rocSOLVER handles (readout unit)
+    int path = 0;               // 0 auto, 1 per-step kernels, 2 persistent kernel
+    unsigned long long* xch = nullptr; size_t xch_words = 0; int xch_plan_G = 0;
+    unsigned* abort_flag = nullptr;
 };
 
 namespace sdev {
@@ -107,4 +110,8 @@ int launch_step(spires_dev* g, const double* u_dev, const double* u_host, double
                 uint8_t* rec_S);
 int launch_output(spires_dev* g, double* y_dev);
 void readout_release(spires_dev* g);
+int persist_supported(spires_dev* g);
+int launch_persist(spires_dev* g, const double* u_series, long long t0, long long T, double* rec_V,
+                   uint8_t* rec_S);
+int outputs_from_states(spires_dev* g, const double* states, size_t T, double* out);
 }  // namespace sdev

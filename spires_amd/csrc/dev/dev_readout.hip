@@ -147,6 +147,18 @@ __global__ void k_online(int n, int dout, double lr, const double* t, const doub
 
 static inline unsigned nb(size_t n, unsigned b) { return (unsigned)((n + b - 1) / b); }
 
+// Y [T x Dout] = X [T x N] W_out' for a recorded state matrix (row-major).
+int outputs_from_states(spires_dev* g, const double* states, size_t T, double* out) {
+    if (g->dout == 0 || T == 0) return 0;
+    rocblas_handle h = get_handle(g);
+    if (!h) { set_error("rocblas_create_handle failed"); return SPIRES_DEV_EINTERNAL; }
+    const double one = 1.0, zero = 0.0;
+    // column-major view: Y' (Dout x T) = W_out (Dout x N, ld N as row-major -> use transpose) * X' (N x T)
+    SDEV_RB(rocblas_dgemm(h, rocblas_operation_transpose, rocblas_operation_none, g->dout, (int)T, g->n, &one,
+                          g->W_out, g->n, states, g->n, &zero, out, g->dout));
+    return 0;
+}
+
 }  // namespace sdev
 
 using namespace sdev;

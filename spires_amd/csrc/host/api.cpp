@@ -634,6 +634,11 @@ spires_status spires_ext_set_mode(spires_reservoir* r, int mode, int block_k) {
     return dev_status(rc);
 }
 
+spires_status spires_ext_set_kernel_path(spires_reservoir* r, int path) {
+    if (!r || !r->dev) return SPIRES_ERR_INVALID_ARG;
+    return dev_status(spires_dev_set_path(r->dev, path));
+}
+
 spires_status spires_ext_set_state(spires_reservoir* r, const double* V) {
     if (!r || !r->dev || !V) return SPIRES_ERR_INVALID_ARG;
     return dev_status(spires_dev_set_state(r->dev, V));

@@ -210,3 +210,43 @@ def test_invalid_inputs_raise_cleanly():
     L = sa.lib()
     assert L.spires_read_reservoir_state(r.h, None) == 1
     assert L.spires_train_ridge(r.h, None, None, 5, 1.0) == 1
+
+
+@pytest.mark.parametrize("path", [sa.PATH_PER_STEP, sa.PATH_PERSISTENT])
+@pytest.mark.parametrize("case", [CASES[0], CASES[2], CASES[5], CASES[7], CASES[11]],
+                         ids=["gl-ws", "gl-ba", "gl-psch", "gl-psche", "lifd"])
+def test_kernel_paths_agree_with_oracle(case, path):
+    """Both exact-mode kernel paths (one launch per micro-step, one persistent
+    launch per run) reproduce the oracle, including across several runs that
+    continue the same state."""
+    print("case", case, "path", path, flush=True)
+    r, o = _pair(case, n=300)
+    r.set_kernel_path(path)
+    u = np.random.default_rng(21).uniform(0, 0.5, (90, 1))
+    for part in (u[:30], u[30:31], u[31:90]):
+        X, R, _ = r.run_record(part)
+        Xo, Ro, _ = o.run_record(part, part.shape[0])
+        assert np.array_equal(R, Ro) and np.array_equal(_bits(X), _bits(Xo))
+    r.step(u[0]); o.step(u[0])
+    assert np.array_equal(_bits(r.state()), _bits(o.state()))
+
+
+def test_persistent_hbm_ring_path():
+    """N*L too large for LDS: the persistent kernel streams the history from HBM."""
+    n, L = 12288, 256
+    params = [1.0, 0.0, 0.0, 20.0, 0.8, float(L), 0.05, 2.0]
+    sa.srand(5)
+    rp, ci, _, _, _ = sa.build_weights(sa.make_config(n, conn=10 / (n - 1), conn_type="small_world",
+                                                      params=params))
+    val = np.random.default_rng(5).normal(0, 1, len(ci))
+    val *= 0.95 / sa.spectral_radius(rp, ci, val)
+    Wi = np.random.default_rng(6).uniform(-1, 1, (n, 1))
+    kw = dict(din=1, dout=1, gain=2.0, neuron="flif_gl", params=params)
+    r = sa.Reservoir(sa.make_config(n, **kw), csr=(rp, ci, val), W_in=Wi)
+    r.set_kernel_path(sa.PATH_PERSISTENT)
+    o = po.OracleReservoir(po.make_config(n, **kw), csr=(rp, ci, val), W_in=Wi)
+    T = 40
+    u = np.random.default_rng(7).uniform(0, 0.5, (T, 1))
+    X, R, _ = r.run_record(u)
+    Xo, Ro, _ = o.run_record(u, T)
+    assert np.array_equal(R, Ro) and np.array_equal(_bits(X), _bits(Xo))
