@@ -544,7 +544,7 @@ int spires_dev_create(const spires_dev_desc* d, spires_dev** out) {
     };
     if ((rc = up(g->W_in, d->W_in, n * g->din * 8))) return fail(rc);
     if (d->W_out) { if ((rc = up(g->W_out, d->W_out, n * g->dout * 8))) return fail(rc); }
-    else if (hipMemsetAsync(g->W_out, 0, n * g->dout * 8 + 8, g->stream) != hipSuccess) return fail(SPIRES_DEV_EINTERNAL);
+    else if (n * g->dout) { if (hipMemsetAsync(g->W_out, 0, n * g->dout * 8, g->stream) != hipSuccess) return fail(hip_fail(hipGetLastError(), "W_out clear", __LINE__)); }
     if (g->backend == BK_SPARSE) {
         g->nnz = d->nnz;
         if ((rc = dalloc(&g->rp, n + 1)) || (rc = dalloc(&g->col, (size_t)d->nnz)) || (rc = dalloc(&g->val, (size_t)d->nnz)))
@@ -571,7 +571,7 @@ int spires_dev_create(const spires_dev_desc* d, spires_dev** out) {
         std::vector<double> dec(m ? m : 1, d->syn_decay);
         if ((rc = dalloc(&g->edge_decay, m)) || (rc = dalloc(&g->edge_tr, m))) return fail(rc);
         if ((rc = up(g->edge_decay, dec.data(), m * 8))) return fail(rc);
-        SDEV_CK(hipMemsetAsync(g->edge_tr, 0, m * 8 + 8, g->stream));
+        if (m) SDEV_CK(hipMemsetAsync(g->edge_tr, 0, m * 8, g->stream));
         SDEV_CK(hipStreamSynchronize(g->stream));
     } else if (g->stype == ST_PSC_HOM) {
         if ((rc = dalloc(&g->tr[0], n)) || (rc = dalloc(&g->tr[1], n))) return fail(rc);
@@ -589,7 +589,7 @@ int spires_dev_create(const spires_dev_desc* d, spires_dev** out) {
             if ((rc = up(g->edge_decay, T.data(), m * 8))) return fail(rc);
             SDEV_CK(hipStreamSynchronize(g->stream));
         }
-        SDEV_CK(hipMemsetAsync(g->edge_tr, 0, m * 8, g->stream));
+        if (m) SDEV_CK(hipMemsetAsync(g->edge_tr, 0, m * 8, g->stream));
     } else if (g->stype == ST_FRACTIONAL) {
         g->frac_L = d->frac_L; g->frac_da = d->frac_dt_alpha; g->frac_tau = d->frac_tau;
         if ((rc = dalloc(&g->frac_coef, (size_t)d->frac_L)) || (rc = dalloc(&g->frac_hist, n * d->frac_L)) ||
@@ -605,7 +605,10 @@ int spires_dev_create(const spires_dev_desc* d, spires_dev** out) {
         SDEV_CK(hipMemsetAsync(g->me_tr, 0, n * d->me_n * 8, g->stream));
     }
     if ((rc = init_neurons(g, d, true))) return fail(rc);
-    if (hipStreamSynchronize(g->stream) != hipSuccess) return fail(hip_fail(hipGetLastError(), "create sync", __LINE__));
+    {
+        hipError_t e = hipStreamSynchronize(g->stream);
+        if (e != hipSuccess) return fail(hip_fail(e, "create sync", __LINE__));
+    }
     *out = g;
     return 0;
 }
