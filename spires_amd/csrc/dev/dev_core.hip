@@ -370,6 +370,8 @@ static int init_neurons(spires_dev* g, const spires_dev_desc* d, bool full) {
         dfree(rng0);
     }
     if (g->xch) SDEV_CK(hipMemsetAsync(g->xch, 0, g->xch_words * 8, g->stream));
+    g->ring32_valid = 0;
+    g->ring64_valid = 1;
     g->cur = 0;
     g->step = 0;
     g->msteps = 0;
@@ -378,6 +380,12 @@ static int init_neurons(spires_dev* g, const spires_dev_desc* d, bool full) {
 
 int launch_step(spires_dev* g, const double* u_dev, const double* u_host, double* rec_V,
                 uint8_t* rec_S) {
+    if (!g->ring64_valid) {
+        int rcs = fast_sync_rings(g, false);
+        if (rcs) return rcs;
+        g->ring64_valid = 1;
+    }
+    g->ring32_valid = 0;
     StepArgs a;
     memset(&a, 0, sizeof(a));
     a.n = g->n; a.din = g->din; a.L = g->L; a.m = g->m;
@@ -498,7 +506,7 @@ void spires_dev_destroy(spires_dev* g) {
     dfree(g->edge_decay); dfree(g->edge_tr); dfree(g->frac_coef); dfree(g->frac_hist);
     dfree(g->frac_I); dfree(g->me_tr); dfree(g->me_comb); dfree(g->u_series);
     dfree(g->rec_state); dfree(g->rec_raster); dfree(g->rec_out); dfree(g->ybuf); dfree(g->ustep);
-    dfree(g->xch); dfree(g->abort_flag);
+    dfree(g->xch); dfree(g->abort_flag); dfree(g->ring32);
     if (g->stream) (void)hipStreamDestroy(g->stream);
     delete g;
 }
@@ -642,6 +650,25 @@ int spires_dev_upload_inputs(spires_dev* g, const double* u, size_t T) {
 int spires_dev_run(spires_dev* g, size_t t0, size_t T, unsigned flags) {
     if (!g) return SPIRES_DEV_EINVAL;
     if (t0 + T > g->u_T && g->din > 0) { set_error("run beyond uploaded inputs"); return SPIRES_DEV_EINVAL; }
+    if (g->mode == SPIRES_DEV_MODE_FAST) {
+        if (!fast_supported(g)) { set_error("fast mode supports FLIF-GL with SIMPLE/PSC sparse synapses only"); return SPIRES_DEV_EINVAL; }
+        unsigned needf = flags | ((flags & SPIRES_DEV_REC_OUTPUT) ? SPIRES_DEV_REC_STATE : 0u);
+        int rcf = ensure_records(g, T, needf);
+        if (rcf) return rcf;
+        if (T == 0) return 0;
+        const double* u = g->din > 0 ? g->u_series : nullptr;
+        if ((rcf = launch_fast(g, u, (long long)t0, (long long)T, (needf & SPIRES_DEV_REC_STATE) ? g->rec_state : nullptr,
+                               (flags & SPIRES_DEV_REC_RASTER) ? g->rec_raster : nullptr)))
+            return rcf;
+        if (flags & SPIRES_DEV_REC_OUTPUT) return outputs_from_states(g, g->rec_state, T, g->rec_out);
+        return 0;
+    }
+    if (!g->ring64_valid) {
+        int rcs = fast_sync_rings(g, false);
+        if (rcs) return rcs;
+        g->ring64_valid = 1;
+    }
+    g->ring32_valid = 0;
     const bool persist = g->mode == SPIRES_DEV_MODE_EXACT && g->path != 1 && T > 0 && persist_supported(g);
     if (g->path == 2 && !persist) { set_error("persistent kernel cannot run this reservoir"); return SPIRES_DEV_EINVAL; }
     unsigned need = flags;

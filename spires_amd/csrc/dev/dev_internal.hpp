@@ -103,6 +103,8 @@ struct spires_dev {
     int path = 0;               // 0 auto, 1 per-step kernels, 2 persistent kernel
     unsigned long long* xch = nullptr; size_t xch_words = 0; int xch_plan_G = 0;
     unsigned* abort_flag = nullptr;
+    float* ring32 = nullptr; size_t ring32_cap = 0;   // FP32 history of the fast mode
+    int ring32_valid = 0, ring64_valid = 1;
 };
 
 namespace sdev {
@@ -114,4 +116,8 @@ int persist_supported(spires_dev* g);
 int launch_persist(spires_dev* g, const double* u_series, long long t0, long long T, double* rec_V,
                    uint8_t* rec_S);
 int outputs_from_states(spires_dev* g, const double* states, size_t T, double* out);
+int fast_supported(spires_dev* g);
+int fast_sync_rings(spires_dev* g, bool to_fast);
+int launch_fast(spires_dev* g, const double* u_series, long long t0, long long T, double* rec_V,
+                uint8_t* rec_S);
 }  // namespace sdev
