@@ -555,6 +555,7 @@ int spires_dev_create(const spires_dev_desc* d, spires_dev** out) {
     else if (n * g->dout) { if (hipMemsetAsync(g->W_out, 0, n * g->dout * 8, g->stream) != hipSuccess) return fail(hip_fail(hipGetLastError(), "W_out clear", __LINE__)); }
     if (g->backend == BK_SPARSE) {
         g->nnz = d->nnz;
+        g->rp_host.assign(d->row_ptr, d->row_ptr + n + 1);
         if ((rc = dalloc(&g->rp, n + 1)) || (rc = dalloc(&g->col, (size_t)d->nnz)) || (rc = dalloc(&g->val, (size_t)d->nnz)))
             return fail(rc);
         if ((rc = up(g->rp, d->row_ptr, (n + 1) * 8)) || (rc = up(g->col, d->col, (size_t)d->nnz * 4)) ||

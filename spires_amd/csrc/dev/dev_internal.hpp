@@ -4,6 +4,7 @@
 #include <cstdint>
 #include <cstddef>
 #include <string>
+#include <vector>
 
 #include "spires_dev.h"
 
@@ -78,6 +79,7 @@ struct spires_dev {
     // weights
     double* W_in = nullptr; double* W_out = nullptr;
     int64_t* rp = nullptr; int32_t* col = nullptr; double* val = nullptr; int64_t nnz = 0;
+    std::vector<int64_t> rp_host;   // host copy of the row pointers (launch planning)
     double* WT = nullptr;       // dense backend, transposed (column j contiguous over rows)
     // neuron state
     double* coef = nullptr; int ncoef = 0;
