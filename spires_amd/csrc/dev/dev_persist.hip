@@ -56,35 +56,39 @@ struct PersistArgs {
 };
 
 // h += sum_{q<cnt} c[kb+q] * ring[(sb-q)*STRIDE + lane]   (descending slots, ascending lags)
-// Blocks of 16 lags; the next block's loads are issued before the current
-// block's dependent adds.
+// Blocks of 16 lags in two register sets used alternately (no copies), so the
+// loads of the next block are in flight while the current block's dependent
+// adds run.
+template <int STRIDE>
+__device__ __forceinline__ void gl_load16(const double* __restrict__ ring, const double* __restrict__ c, int kb,
+                                          int sb, int lane, double* v, double* w) {
+    const double* rb = ring + (sb - 15) * STRIDE + lane;
+#pragma unroll
+    for (int j = 0; j < 16; ++j) { v[j] = rb[(15 - j) * STRIDE]; w[j] = c[kb + j]; }
+}
+__device__ __forceinline__ double gl_acc16(double h, const double* v, const double* w) {
+#pragma unroll
+    for (int j = 0; j < 16; ++j) h = h + w[j] * v[j];
+    return h;
+}
 template <int STRIDE>
 __device__ __forceinline__ double gl_seg_lds(double h, const double* __restrict__ ring, const double* __restrict__ c,
                                              int kb, int sb, int cnt, int lane) {
-    constexpr int B = 16;
-    int q = 0;
-    if (cnt >= B) {
-        double v0[B], w0[B];
-        {
-            const double* rb = ring + (sb - (B - 1)) * STRIDE + lane;
-#pragma unroll
-            for (int j = 0; j < B; ++j) { v0[j] = rb[(B - 1 - j) * STRIDE]; w0[j] = c[kb + j]; }
+    const int nblk = cnt >> 4;
+    int b = 0;
+    if (nblk > 0) {
+        double va[16], wa[16], vb[16], wb[16];
+        gl_load16<STRIDE>(ring, c, kb, sb, lane, va, wa);
+        while (b + 2 <= nblk) {
+            gl_load16<STRIDE>(ring, c, kb + 16 * (b + 1), sb - 16 * (b + 1), lane, vb, wb);
+            h = gl_acc16(h, va, wa);
+            if (b + 2 < nblk) gl_load16<STRIDE>(ring, c, kb + 16 * (b + 2), sb - 16 * (b + 2), lane, va, wa);
+            h = gl_acc16(h, vb, wb);
+            b += 2;
         }
-        for (; q + 2 * B <= cnt; q += B) {
-            double v1[B], w1[B];
-            const double* rb = ring + (sb - q - B - (B - 1)) * STRIDE + lane;
-#pragma unroll
-            for (int j = 0; j < B; ++j) { v1[j] = rb[(B - 1 - j) * STRIDE]; w1[j] = c[kb + q + B + j]; }
-#pragma unroll
-            for (int j = 0; j < B; ++j) h = h + w0[j] * v0[j];
-#pragma unroll
-            for (int j = 0; j < B; ++j) { v0[j] = v1[j]; w0[j] = w1[j]; }
-        }
-#pragma unroll
-        for (int j = 0; j < B; ++j) h = h + w0[j] * v0[j];
-        q += B;
+        if (b < nblk) { h = gl_acc16(h, va, wa); ++b; }
     }
-    for (; q < cnt; ++q) h = h + c[kb + q] * ring[(sb - q) * STRIDE + lane];
+    for (int q = 16 * nblk; q < cnt; ++q) h = h + c[kb + q] * ring[(sb - q) * STRIDE + lane];
     return h;
 }
 
@@ -153,9 +157,16 @@ __global__ void __launch_bounds__(kPersistThreads) k_persist(PersistArgs a) {
     double tp = (own && GL) ? a.tprev[i] : 0.0;
     bool spk = own ? (a.S[i] != 0) : false;
     int rs = 0, re = 0;
+    constexpr int kMaxDinReg = 16;
+    double win[kMaxDinReg];
+    const bool win_reg = a.din <= kMaxDinReg;
+#pragma unroll
+    for (int j = 0; j < kMaxDinReg; ++j) win[j] = 0.0;
     if (wave == 1 && lane < nloc) {
         rs = (int)(a.rp[i] - E0);
         re = (int)(a.rp[i + 1] - E0);
+        if (win_reg)
+            for (int j = 0; j < a.din; ++j) win[j] = a.W_in[(size_t)i * a.din + j];
     }
     const double* p = a.np;
     __syncthreads();
@@ -183,6 +194,7 @@ __global__ void __launch_bounds__(kPersistThreads) k_persist(PersistArgs a) {
                 }
                 __syncthreads();   // rec / ext of this step ready
                 if (abort_s) break;
+                int ring_slot_write = 0;
                 if (own) {
                     const double x = ext_s[tb * NB + lane] + rec_s[sb * NB + lane];
                     spk = false;
@@ -203,6 +215,19 @@ __global__ void __launch_bounds__(kPersistThreads) k_persist(PersistArgs a) {
                         } else {
                             V = p[1];
                         }
+                        ring_slot_write = 1;
+                    }
+                }
+                // publish first (the other workgroups wait on it): word = [tag | bits of 32 neurons]
+                const unsigned long long m = __ballot(own && spk);
+                if (lane < wpg) {
+                    const unsigned bits = (unsigned)(m >> (32 * lane));
+                    const unsigned long long word = ((unsigned long long)(unsigned)(st + 1) << 32) | bits;
+                    __hip_atomic_store(a.xch + (size_t)sb * nw + (size_t)g * wpg + lane, word, __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT);
+                }
+                if (own) {
+                    if (GL && ring_slot_write) {
                         if (RL) ring_s[head * NB + lane] = V;
                         else a.ring[(size_t)head * a.n + i] = V;
                         tp = tp + a.dt;
@@ -212,15 +237,13 @@ __global__ void __launch_bounds__(kPersistThreads) k_persist(PersistArgs a) {
                         if (a.rec_S) a.rec_S[(size_t)t * a.n + i] = spk ? 1 : 0;
                     }
                 }
-                // publish: word = [tag | bits of 32 neurons]
-                const unsigned long long m = __ballot(own && spk);
-                if (lane < wpg) {
-                    const unsigned bits = (unsigned)(m >> (32 * lane));
-                    const unsigned long long word = ((unsigned long long)(unsigned)(st + 1) << 32) | bits;
-                    __hip_atomic_store(a.xch + (size_t)sb * nw + (size_t)g * wpg + lane, word, __ATOMIC_RELAXED,
-                                       __HIP_MEMORY_SCOPE_AGENT);
-                }
             } else {
+                // prefetch this macro step's input (its latency overlaps the exchange wait)
+                double uin[kMaxDinReg];
+                if (mi == 0 && win_reg) {
+#pragma unroll
+                    for (int j = 0; j < kMaxDinReg; ++j) uin[j] = (j < a.din) ? a.u[tt * a.din + j] : 0.0;
+                }
                 // ---------------- spikes of micro-step st-1
                 bool aborted = false;
                 if (st == a.st0) {
@@ -236,31 +259,38 @@ __global__ void __launch_bounds__(kPersistThreads) k_persist(PersistArgs a) {
                 } else {
                     const unsigned long long* slot = a.xch + (size_t)(sb ^ 1) * nw;
                     const unsigned tag = (unsigned)st;       // tag of micro-step st-1
-                    for (int w0 = 0; w0 < nw && !aborted; w0 += 64) {
-                        const int w = w0 + lane;
-                        bool ready = w >= nw;
-                        unsigned long long word = 0;
-                        long long spins = 0;
-                        while (true) {
-                            if (!ready) {
-                                word = ld_word(&slot[w]);
-                                ready = (unsigned)(word >> 32) == tag;
-                            }
-                            if (__all(ready)) break;
-                            const bool give_up = (++spins > kSpinLimit) ||
-                                __hip_atomic_load(a.abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u;
-                            if (__any(give_up)) {
-                                aborted = true;
-                                if (lane == 0) {
-                                    abort_s = 1;
-                                    __hip_atomic_store(a.abort_flag, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                                }
-                                break;
-                            }
-                            __builtin_amdgcn_s_sleep(1);
+                    constexpr int kMaxWordsPerLane = 8;      // nw <= 512
+                    unsigned long long word[kMaxWordsPerLane];
+                    bool ready[kMaxWordsPerLane];
+#pragma unroll
+                    for (int q = 0; q < kMaxWordsPerLane; ++q) { ready[q] = lane + 64 * q >= nw; word[q] = 0; }
+                    long long spins = 0;
+                    while (true) {
+#pragma unroll
+                        for (int q = 0; q < kMaxWordsPerLane; ++q)
+                            if (!ready[q]) word[q] = ld_word(&slot[lane + 64 * q]);
+                        bool all_here = true;
+#pragma unroll
+                        for (int q = 0; q < kMaxWordsPerLane; ++q) {
+                            if (!ready[q]) ready[q] = (unsigned)(word[q] >> 32) == tag;
+                            all_here = all_here && ready[q];
                         }
-                        if (w < nw) bits_s[w] = (unsigned)word;
+                        if (__all(all_here)) break;
+                        const bool give_up = (++spins > kSpinLimit) ||
+                            __hip_atomic_load(a.abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u;
+                        if (__any(give_up)) {
+                            aborted = true;
+                            if (lane == 0) {
+                                abort_s = 1;
+                                __hip_atomic_store(a.abort_flag, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                            }
+                            break;
+                        }
+                        __builtin_amdgcn_s_sleep(1);
                     }
+#pragma unroll
+                    for (int q = 0; q < kMaxWordsPerLane; ++q)
+                        if (lane + 64 * q < nw) bits_s[lane + 64 * q] = (unsigned)word[q];
                 }
                 __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
                 __builtin_amdgcn_wave_barrier();
@@ -270,7 +300,20 @@ __global__ void __launch_bounds__(kPersistThreads) k_persist(PersistArgs a) {
                     double s = 0.0;
                     if (a.stype == ST_SIMPLE) {
                         if (csr_lds) {
-                            for (int e = rs; e < re; ++e) {
+                            // batches of 8 edges: all LDS loads issued, then the in-order sum
+                            int e = rs;
+                            for (; e + 8 <= re; e += 8) {
+                                unsigned pk[8], bw[8];
+                                double wv[8];
+#pragma unroll
+                                for (int q = 0; q < 8; ++q) { pk[q] = eidx_s[e + q]; wv[q] = eval_s[e + q]; }
+#pragma unroll
+                                for (int q = 0; q < 8; ++q) bw[q] = bits_s[pk[q] >> 5];
+#pragma unroll
+                                for (int q = 0; q < 8; ++q)
+                                    if ((bw[q] >> (pk[q] & 31)) & 1u) s = s + wv[q];
+                            }
+                            for (; e < re; ++e) {
                                 const unsigned pk = eidx_s[e];
                                 if ((bits_s[pk >> 5] >> (pk & 31)) & 1u) s = s + eval_s[e];
                             }
@@ -304,7 +347,13 @@ __global__ void __launch_bounds__(kPersistThreads) k_persist(PersistArgs a) {
                     rec_s[sb * NB + lane] = s;
                     if (mi == 0) {
                         double su = 0.0;
-                        for (int j = 0; j < a.din; ++j) su = su + a.W_in[(size_t)i * a.din + j] * a.u[tt * a.din + j];
+                        if (win_reg) {
+#pragma unroll
+                            for (int j = 0; j < kMaxDinReg; ++j)
+                                if (j < a.din) su = su + win[j] * uin[j];
+                        } else {
+                            for (int j = 0; j < a.din; ++j) su = su + a.W_in[(size_t)i * a.din + j] * a.u[tt * a.din + j];
+                        }
                         ext_s[tb * NB + lane] = a.gain * su;
                     }
                 }
@@ -351,8 +400,8 @@ bool persist_plan(spires_dev* g, int ncu, Plan* pl) {
     };
     auto fill = [&](int nb, int rl) -> bool {
         const int G = (n + nb - 1) / nb;
-        if (G > ncu) return false;
         const int wpg = (nb + 31) / 32;
+        if (G > ncu || G * wpg > 512) return false;
         size_t base = (rl ? (size_t)L * nb * 8 : 0) + (gl ? (size_t)L * 8 : 0) + 4 * (size_t)nb * 8 +
                       (size_t)G * wpg * 4 + 64;
         if (base > cap) return false;

@@ -1,7 +1,9 @@
 // Readout training on the device.
 //
 // Ridge (reservoir.c:573-679): the recorded state matrix X [T x N] stays in
-// HBM.  A = X'X (+ lambda I) and B = X'Y are formed on the GPU, then one FP64
+// HBM.  A = X'X (+ lambda I) and B = X'Y are formed on the GPU (in fast mode
+// the Gram comes from split-BF16 products hi'hi + hi'lo + lo'hi accumulated in
+// FP32 on the matrix cores and combined in FP64), then one FP64
 // Cholesky factorisation solves all Dout right-hand sides at once (the
 // reference refactorises A with LU once per output), followed by FP64
 // iterative refinement against the unfactored A.  A matrix that is not
@@ -60,6 +62,34 @@ __global__ void k_ridge_diag_sym(int n, double* A, double lambda) {
     const int c = (int)(idx / n), r = (int)(idx % n);   // column-major (r, c)
     if (r == c) A[idx] = A[idx] + lambda;
     else if (r < c) A[idx] = A[(size_t)r * n + c];      // upper <- lower (c, r)
+}
+
+// Split of an FP64 matrix into bf16 hi + lo parts: hi = bf16(x), lo = bf16(x - hi),
+// round-to-nearest-even on the FP32 value (|x - hi - lo| <~ 2^-16 |x|).
+__device__ __forceinline__ unsigned short f32_to_bf16(float f) {
+    const unsigned u = __float_as_uint(f);
+    const unsigned r = ((u >> 16) & 1u) + 0x7fffu;
+    return (unsigned short)((u + r) >> 16);
+}
+__device__ __forceinline__ float bf16_to_f32(unsigned short h) { return __uint_as_float((unsigned)h << 16); }
+
+__global__ void k_split_bf16(size_t cnt, const double* x, unsigned short* hi, unsigned short* lo) {
+    const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= cnt) return;
+    const double v = x[q];
+    const unsigned short h = f32_to_bf16((float)v);
+    hi[q] = h;
+    lo[q] = f32_to_bf16((float)(v - (double)bf16_to_f32(h)));
+}
+
+// A = G1 + C + C' (+ lambda on the diagonal), FP32 products combined in FP64.
+__global__ void k_combine_gram(int n, const float* G1, const float* Cx, double lambda, double* A) {
+    const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (size_t)n * n) return;
+    const int c = (int)(idx / n), r = (int)(idx % n);
+    double v = (double)G1[idx] + (double)Cx[idx] + (double)Cx[(size_t)r * n + c];
+    if (r == c) v += lambda;
+    A[idx] = v;
 }
 
 __global__ void k_axpy(size_t n, const double* x, double* y) {
@@ -189,16 +219,45 @@ int spires_dev_ridge(spires_dev* g, size_t washout, const double* Y, double lamb
     const double* Xs = g->rec_state + washout * (size_t)n;   // column-major n x T (ld n) == X'
     const double one = 1.0, zero = 0.0, mone = -1.0;
     H(hipMemcpyAsync(Yd, Y, (size_t)T * D * 8, hipMemcpyHostToDevice, g->stream));
-    // A = X'X (lower), B = X'Y
-    rocblas_status s1 = rocblas_dsyrk(h, rocblas_fill_lower, rocblas_operation_none, n, T, &one, Xs, n, &zero, A, n);
+    // B = X'Y (FP64)
     rocblas_status s2 = rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_transpose, n, D, T, &one, Xs, n,
                                       Yd, D, &zero, B, n);
+    rocblas_status s1 = rocblas_status_success;
+    if (g->mode == SPIRES_DEV_MODE_FAST) {
+        // Gram from split-BF16 products with FP32 accumulation: X'X ~ hi'hi + hi'lo + lo'hi
+        unsigned short *hi = nullptr, *lo = nullptr;
+        float *G1 = nullptr, *Cx = nullptr;
+        const size_t cnt = (size_t)n * T;
+        int rc2 = 0;
+        if ((rc2 = dalloc(&hi, cnt)) || (rc2 = dalloc(&lo, cnt)) || (rc2 = dalloc(&G1, nn)) || (rc2 = dalloc(&Cx, nn))) {
+            dfree(hi); dfree(lo); dfree(G1); dfree(Cx);
+            cleanup();
+            return rc2;
+        }
+        hipLaunchKernelGGL(k_split_bf16, dim3(nb(cnt, 256)), dim3(256), 0, g->stream, cnt, Xs, hi, lo);
+        const float onef = 1.0f, zerof = 0.0f;
+        s1 = rocblas_gemm_ex(h, rocblas_operation_none, rocblas_operation_transpose, n, n, T, &onef, hi,
+                             rocblas_datatype_bf16_r, n, hi, rocblas_datatype_bf16_r, n, &zerof, G1,
+                             rocblas_datatype_f32_r, n, G1, rocblas_datatype_f32_r, n, rocblas_datatype_f32_r,
+                             rocblas_gemm_algo_standard, 0, 0);
+        if (s1 == rocblas_status_success)
+            s1 = rocblas_gemm_ex(h, rocblas_operation_none, rocblas_operation_transpose, n, n, T, &onef, hi,
+                                 rocblas_datatype_bf16_r, n, lo, rocblas_datatype_bf16_r, n, &zerof, Cx,
+                                 rocblas_datatype_f32_r, n, Cx, rocblas_datatype_f32_r, n, rocblas_datatype_f32_r,
+                                 rocblas_gemm_algo_standard, 0, 0);
+        hipLaunchKernelGGL(k_combine_gram, dim3(nb(nn, 256)), dim3(256), 0, g->stream, n, G1, Cx, lambda, A);
+        H(hipStreamSynchronize(g->stream));
+        dfree(hi); dfree(lo); dfree(G1); dfree(Cx);
+    } else {
+        // A = X'X (lower) in FP64
+        s1 = rocblas_dsyrk(h, rocblas_fill_lower, rocblas_operation_none, n, T, &one, Xs, n, &zero, A, n);
+        hipLaunchKernelGGL(k_ridge_diag_sym, dim3(nb(nn, 256)), dim3(256), 0, g->stream, n, A, lambda);
+    }
     if (s1 != rocblas_status_success || s2 != rocblas_status_success) {
         cleanup();
         set_error("Gram/cross products failed");
         return SPIRES_DEV_EINTERNAL;
     }
-    hipLaunchKernelGGL(k_ridge_diag_sym, dim3(nb(nn, 256)), dim3(256), 0, g->stream, n, A, lambda);
     H(hipMemcpyAsync(A0, A, nn * 8, hipMemcpyDeviceToDevice, g->stream));
     H(hipMemcpyAsync(X, B, nd * 8, hipMemcpyDeviceToDevice, g->stream));
     rocblas_status s3 = rocsolver_dpotrf(h, rocblas_fill_lower, n, A, n, info);

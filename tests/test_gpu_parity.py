@@ -255,14 +255,14 @@ def test_persistent_hbm_ring_path():
     assert np.array_equal(R, Ro) and np.array_equal(_bits(X), _bits(Xo))
 
 
-def _gl_gauss(n, L, seed, topo="small_world", k=20):
+def _gl_gauss(n, L, seed, topo="small_world", k=20, dout=2):
     params = [1.0, 0.0, 0.0, 20.0, 0.8, float(L), 0.05, 2.0]
     sa.srand(seed)
     rp, ci, _, _, _ = sa.build_weights(sa.make_config(n, conn=k / (n - 1), conn_type=topo, params=params))
     val = np.random.default_rng(seed).normal(0, 1, len(ci))
     val *= 0.95 / sa.spectral_radius(rp, ci, val)
     Wi = np.random.default_rng(seed + 1).uniform(-1, 1, (n, 1))
-    kw = dict(din=1, dout=2, gain=2.0, neuron="flif_gl", params=params)
+    kw = dict(din=1, dout=dout, gain=2.0, neuron="flif_gl", params=params)
     return kw, (rp, ci, val), Wi
 
 
@@ -309,3 +309,31 @@ def test_mode_switching_keeps_state():
     r.set_mode(sa.MODE_EXACT, 0)
     X2, R2, _ = r.run_record(u[200:300])
     assert np.isfinite(X2).all() and R2.sum() > 0
+
+
+def _nrmse(y, yh):
+    return float(np.sqrt(np.mean((y - yh) ** 2)) / np.std(y))
+
+
+def test_fast_ridge_split_bf16_matches_fp64_readout():
+    """Fast mode (FP32 history, split-BF16 Gram, FP64 Cholesky + refinement)
+    against the FP64 exact path on a delayed-recall task: validation NRMSE of
+    the two readouts within 0.02 of each other."""
+    n, L, T, wo, Tv = 512, 128, 4000, 200, 1000
+    kw, csr, Wi = _gl_gauss(n, L, 31, dout=4)
+    rng = np.random.default_rng(32)
+    u = rng.uniform(0, 0.5, (T + Tv, 1))
+    delays = [1, 2, 4, 8]
+    y = np.zeros((T + Tv, 4))
+    for j, d in enumerate(delays):
+        y[d:, j] = u[:-d, 0]
+    scores = {}
+    for mode in (sa.MODE_EXACT, sa.MODE_FAST):
+        r = sa.Reservoir(sa.make_config(n, **kw), csr=csr, W_in=Wi)
+        r.set_mode(mode, 32)
+        r.train_ridge_washout(u[:T], y[wo:T], wo, 1e-5)
+        _, _, Yh = r.run_record(u[T:], states=False, raster=False, outputs=True)
+        scores[mode] = _nrmse(y[T:], Yh)
+    print("NRMSE exact", scores[sa.MODE_EXACT], "fast", scores[sa.MODE_FAST])
+    assert scores[sa.MODE_EXACT] < 0.9
+    assert abs(scores[sa.MODE_FAST] - scores[sa.MODE_EXACT]) < 0.02
