@@ -110,7 +110,9 @@ int  spires_dev_online(spires_dev *g, const double *target, double lr);
 
 int  spires_dev_set_mode(spires_dev *g, int mode, int block_k);
 /* Kernel path of exact-mode runs: 0 automatic (persistent kernel when the
- * model supports it), 1 one launch per micro-step, 2 persistent only. */
+ * model supports it), 1 one launch per micro-step, 2 persistent only,
+ * 3 persistent with a single-wave history sum streamed from HBM,
+ * 4 persistent with a single-wave history sum over LDS. */
 int  spires_dev_set_path(spires_dev *g, int path);
 int  spires_dev_sync(spires_dev *g);
 void *spires_dev_stream(spires_dev *g);

@@ -71,7 +71,10 @@ spires_status spires_ext_train_ridge_washout(spires_reservoir *r, const double *
 #define SPIRES_EXT_MODE_FAST  1   /* FP32 time-blocked GL history on matrix cores          */
 spires_status spires_ext_set_mode(spires_reservoir *r, int mode, int block_k);
 /* Exact-mode kernel path: 0 automatic (one persistent launch per run when the
- * model supports it), 1 one launch per micro-step, 2 persistent only. */
+ * model supports it), 1 one launch per micro-step, 2 persistent only (FLIF-GL:
+ * chain wave fed by product waves when it fits), 3 persistent with a
+ * single-wave history sum streamed from HBM, 4 persistent with a single-wave
+ * history sum over an LDS-resident history.  All paths give identical results. */
 spires_status spires_ext_set_kernel_path(spires_reservoir *r, int path);
 
 spires_status spires_ext_set_state(spires_reservoir *r, const double *V);

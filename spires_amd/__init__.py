@@ -25,7 +25,7 @@ STATUS = {0: "SPIRES_OK", 1: "SPIRES_ERR_INVALID_ARG", 2: "SPIRES_ERR_ALLOC", 3:
 
 REC_STATE, REC_RASTER, REC_OUTPUT = 1, 2, 4
 MODE_EXACT, MODE_FAST = 0, 1
-PATH_AUTO, PATH_PER_STEP, PATH_PERSISTENT = 0, 1, 2
+PATH_AUTO, PATH_PER_STEP, PATH_PERSISTENT, PATH_PERSISTENT_HBM, PATH_PERSISTENT_LDS = 0, 1, 2, 3, 4
 
 
 class SpiresError(RuntimeError):

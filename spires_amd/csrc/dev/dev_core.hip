@@ -671,7 +671,7 @@ int spires_dev_run(spires_dev* g, size_t t0, size_t T, unsigned flags) {
     }
     g->ring32_valid = 0;
     const bool persist = g->mode == SPIRES_DEV_MODE_EXACT && g->path != 1 && T > 0 && persist_supported(g);
-    if (g->path == 2 && !persist) { set_error("persistent kernel cannot run this reservoir"); return SPIRES_DEV_EINVAL; }
+    if (g->path >= 2 && !persist) { set_error("persistent kernel cannot run this reservoir"); return SPIRES_DEV_EINVAL; }
     unsigned need = flags;
     if (persist && (flags & SPIRES_DEV_REC_OUTPUT)) need |= SPIRES_DEV_REC_STATE;   // outputs from the state record
     int rc = ensure_records(g, T, need);
@@ -770,7 +770,7 @@ int spires_dev_set_mode(spires_dev* g, int mode, int block_k) {
 }
 
 int spires_dev_set_path(spires_dev* g, int path) {
-    if (!g || path < 0 || path > 2) return SPIRES_DEV_EINVAL;
+    if (!g || path < 0 || path > 4) return SPIRES_DEV_EINVAL;
     g->path = path;
     return 0;
 }

@@ -13,7 +13,9 @@
 // the j-term triangle summed step-wise from LDS.  The membrane update,
 // threshold comparison and reset stay FP64 (V = pow(dt,alpha) rhs - H).
 #include "dev_internal.hpp"
+#include "dev_exchange.hpp"
 
+#include <algorithm>
 #include <cstring>
 
 namespace sdev {
@@ -22,10 +24,10 @@ typedef float floatx4 __attribute__((ext_vector_type(4)));
 
 constexpr int kFastThreads = 256;     // 4 waves: GEMM on all, steps on waves 0-1
 constexpr int kFastNb = 64;           // neurons per workgroup (one wave of lanes)
-constexpr long long kFastSpinLimit = 1ll << 22;
 
 struct FastArgs {
     int n, din, L, nb, G, wpg, micro, K, stype;
+    int ne_lds;                   // LDS capacity for CSR edges per workgroup (0: CSR from HBM)
     long long st0, t0, T;
     double gain, dt;
     double np[16];
@@ -41,31 +43,53 @@ struct FastArgs {
     double* rec_V; uint8_t* rec_S;
 };
 
-__device__ __forceinline__ unsigned long long fld_word(const unsigned long long* p) {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
 __global__ void __launch_bounds__(kFastThreads) k_fast(FastArgs a) {
-    extern __shared__ float flds[];
-    const int g = blockIdx.x, tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    extern __shared__ __align__(16) float flds[];   // 16-byte aligned: b128 LDS reads
+    // wave index and abort flag are read through readfirstlane so the compiler
+    // treats them as wave-uniform (scalar branches, scalar loop counters).
+    const int g = blockIdx.x, tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int nb = a.nb, i0 = g * nb, nloc = min(nb, a.n - i0), L = a.L, K = a.K;
-    const int nw = a.G * a.wpg;
-    float* c_s = flds;                         // [L] coefficients (padded to even length)
-    float* O_s = c_s + (L + (L & 1));          // [K][nb] pre-block contraction
-    float* Vb_s = O_s + K * nb;                // [K][nb] in-block voltages
-    double* dl = (double*)(Vb_s + K * nb);     // 8-byte aligned: K*nb*4*2 + L*4 (L even)
-    double* rec_s = dl;                        // [nb]
-    double* ext_s = rec_s + nb;                // [nb]
-    double* H_s = ext_s + nb;                  // [nb]
-    unsigned* bits_s = (unsigned*)(H_s + nb);  // [nw]
+    const int wpg = a.wpg, nw = a.G * wpg;
+    // LDS: coefficients (FP32, padded to even length), pre-block contraction O [K][nb],
+    // in-block voltages Vb [K][nb], then 8-byte data: rec [2][nb], ext [2][nb],
+    // CSR values [ne_lds], then 4-byte data: spike bits [nw], CSR columns [ne_lds]
+    float* c_s = flds;
+    float* O_s = c_s + (L + (L & 1));
+    float* Vb_s = O_s + K * nb;
+    double* rec_s = (double*)(Vb_s + K * nb);
+    double* ext_s = rec_s + 2 * nb;
+    double* eval_s = ext_s + 2 * nb;
+    unsigned* bits_s = (unsigned*)(eval_s + a.ne_lds);
+    unsigned* eidx_s = bits_s + nw;
     __shared__ int abort_s;
     if (tid == 0) abort_s = 0;
     for (int k = tid; k < L; k += kFastThreads) c_s[k] = (float)a.coef[k];
+    const int64_t E0 = a.rp[i0];
+    const int ne = (int)(a.rp[i0 + nloc] - E0);
+    const bool csr_lds = a.ne_lds > 0;
+    if (csr_lds)
+        for (int e = tid; e < ne; e += kFastThreads) {
+            eidx_s[e] = encode_col(a.col[E0 + e], nb, wpg);
+            eval_s[e] = a.val[E0 + e];
+        }
     const bool own = wave == 0 && lane < nloc;
     const int i = i0 + lane;
     double V = own ? a.V[i] : 0.0;
     double tp = own ? a.tprev[i] : 0.0;
     bool spk = own ? (a.S[i] != 0) : false;
+    int rs = 0, re = 0;
+    constexpr int kMaxDinReg = 16;
+    double win[kMaxDinReg];
+    const bool win_reg = a.din <= kMaxDinReg;
+#pragma unroll
+    for (int j = 0; j < kMaxDinReg; ++j) win[j] = 0.0;
+    if (wave == 1 && lane < nloc) {
+        rs = (int)(a.rp[i] - E0);
+        re = (int)(a.rp[i + 1] - E0);
+        if (win_reg)
+            for (int j = 0; j < a.din; ++j) win[j] = a.W_in[(size_t)i * a.din + j];
+    }
     const double* p = a.np;
     __syncthreads();
 
@@ -118,100 +142,33 @@ __global__ void __launch_bounds__(kFastThreads) k_fast(FastArgs a) {
             }
         }
         __syncthreads();
-        // ---------------- K steps ----------------
+        // ---------------- K steps (one workgroup barrier each) ----------------
         for (int j = 0; j < K && t < a.T; ++j) {
             const long long tt = a.t0 + t;
             const int head = (int)(st % L);
+            const int sb = (int)(st & 1), tb = (int)(t & 1);
             if (wave == 0) {
+                double H = 0.0;
                 if (own && tp > p[6]) {
                     float h = O_s[j * nb + lane];
                     for (int k = 1; k <= j; ++k) h = h + c_s[k] * Vb_s[(j - k) * nb + lane];
-                    H_s[lane] = (double)h;
+                    H = (double)h;
                 }
-            } else if (wave == 1) {
-                if (st == a.st0) {
-                    for (int w = lane; w < nw; w += 64) {
-                        const int gg = w / a.wpg, part = w % a.wpg;
-                        unsigned b = 0;
-                        for (int q = 0; q < 32; ++q) {
-                            const int nj = gg * nb + part * 32 + q;
-                            if (part * 32 + q < nb && nj < a.n && a.S[nj]) b |= 1u << q;
-                        }
-                        bits_s[w] = b;
-                    }
-                } else {
-                    const unsigned long long* slot = a.xch + (size_t)((st - 1) & 1) * nw;
-                    const unsigned tag = (unsigned)st;
-                    for (int w0 = 0; w0 < nw; w0 += 64) {
-                        const int w = w0 + lane;
-                        bool ready = w >= nw, aborted = false;
-                        unsigned long long word = 0;
-                        long long spins = 0;
-                        while (true) {
-                            if (!ready) {
-                                word = fld_word(&slot[w]);
-                                ready = (unsigned)(word >> 32) == tag;
-                            }
-                            if (__all(ready)) break;
-                            const bool give_up = (++spins > kFastSpinLimit) ||
-                                __hip_atomic_load(a.abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u;
-                            if (__any(give_up)) {
-                                aborted = true;
-                                if (lane == 0) {
-                                    abort_s = 1;
-                                    __hip_atomic_store(a.abort_flag, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                                }
-                                break;
-                            }
-                            __builtin_amdgcn_s_sleep(1);
-                        }
-                        if (w < nw) bits_s[w] = (unsigned)word;
-                        if (aborted) break;
-                    }
-                }
-                if (mi == 0 && lane < nloc) {
-                    double su = 0.0;
-                    for (int q = 0; q < a.din; ++q) su = su + a.W_in[(size_t)(i0 + lane) * a.din + q] * a.u[tt * a.din + q];
-                    ext_s[lane] = a.gain * su;
-                }
-            }
-            __syncthreads();
-            if (abort_s) break;
-            if (wave == 1 && lane < nloc) {
-                const int ii = i0 + lane;
-                double s = 0.0;
-                const int64_t k0 = a.rp[ii], k1 = a.rp[ii + 1];
-                if (a.stype == ST_SIMPLE) {
-                    for (int64_t k = k0; k < k1; ++k) {
-                        const int jn = a.col[k];
-                        const int gg = jn / nb, r = jn - gg * nb;
-                        if ((bits_s[gg * a.wpg + (r >> 5)] >> (r & 31)) & 1u) s = s + a.val[k];
-                    }
-                } else {
-                    for (int64_t k = k0; k < k1; ++k) {
-                        const int jn = a.col[k];
-                        const int gg = jn / nb, r = jn - gg * nb;
-                        const double sj = (double)((bits_s[gg * a.wpg + (r >> 5)] >> (r & 31)) & 1u);
-                        const double d = a.edge_decay[k];
-                        const double tr = a.edge_tr[k] * d + (1.0 - d) * sj;
-                        a.edge_tr[k] = tr;
-                        s = s + a.val[k] * tr;
-                    }
-                }
-                rec_s[lane] = s;
-            }
-            __syncthreads();
-            if (wave == 0) {
+                __syncthreads();
+                if (__builtin_amdgcn_readfirstlane(abort_s)) break;
                 if (own) {
-                    const double x = ext_s[lane] + rec_s[lane];
+                    const double x = ext_s[tb * nb + lane] + rec_s[sb * nb + lane];
                     spk = false;
                     if (tp > p[6]) {
                         const double rhs = (-(V - p[2]) / p[3]) + x + p[5];
-                        V = p[4] * rhs - H_s[lane];
+                        V = p[4] * rhs - H;
                         if (V >= p[0]) { V = p[1]; spk = true; tp = 0.0; }
                     } else {
                         V = p[1];
                     }
+                }
+                publish_spikes(a.xch, nw, g, wpg, st, lane, __ballot(own && spk));
+                if (own) {
                     tp = tp + a.dt;
                     const float vf = (float)V;
                     Vb_s[j * nb + lane] = vf;
@@ -221,20 +178,43 @@ __global__ void __launch_bounds__(kFastThreads) k_fast(FastArgs a) {
                         if (a.rec_S) a.rec_S[(size_t)t * a.n + i] = spk ? 1 : 0;
                     }
                 }
-                const unsigned long long m = __ballot(own && spk);
-                if (lane < a.wpg) {
-                    const unsigned bits = (unsigned)(m >> (32 * lane));
-                    const unsigned long long word = ((unsigned long long)(unsigned)(st + 1) << 32) | bits;
-                    __hip_atomic_store(a.xch + (size_t)(st & 1) * nw + (size_t)g * a.wpg + lane, word,
-                                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            } else if (wave == 1) {
+                double uin[kMaxDinReg];
+                if (mi == 0 && win_reg) {
+#pragma unroll
+                    for (int q = 0; q < kMaxDinReg; ++q) uin[q] = (q < a.din) ? a.u[tt * a.din + q] : 0.0;
                 }
+                const bool aborted = !gather_spikes(a.xch, nw, nb, wpg, a.n, st, a.st0, a.S, bits_s, a.abort_flag,
+                                                    &abort_s, lane);
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                if (!aborted && lane < nloc) {
+                    rec_s[sb * nb + lane] = row_pull(a.stype, csr_lds, rs, re, E0, eidx_s, eval_s, a.col, a.val,
+                                                     bits_s, nb, wpg, a.edge_decay, a.edge_tr);
+                    if (mi == 0) {
+                        double su = 0.0;
+                        if (win_reg) {
+#pragma unroll
+                            for (int q = 0; q < kMaxDinReg; ++q)
+                                if (q < a.din) su = su + win[q] * uin[q];
+                        } else {
+                            for (int q = 0; q < a.din; ++q) su = su + a.W_in[(size_t)i * a.din + q] * a.u[tt * a.din + q];
+                        }
+                        ext_s[tb * nb + lane] = a.gain * su;
+                    }
+                }
+                __syncthreads();
+                if (__builtin_amdgcn_readfirstlane(abort_s)) break;
+            } else {
+                __syncthreads();
+                if (__builtin_amdgcn_readfirstlane(abort_s)) break;
             }
-            __syncthreads();
             ++st;
             if (++mi == a.micro) { mi = 0; ++t; }
         }
-        if (abort_s) break;
-        __syncthreads();   // ring writes of this block precede the next block's contraction
+        if (__builtin_amdgcn_readfirstlane(abort_s)) break;
+        __syncthreads();   // this block's voltages (Vb, ring) precede the next contraction
     }
     if (own) {
         a.V[i] = V;
@@ -330,7 +310,14 @@ int launch_fast(spires_dev* g, const double* u_series, long long t0, long long T
     a.xch = g->xch; a.abort_flag = g->abort_flag;
     a.rec_V = rec_V; a.rec_S = rec_S;
     const int Le = g->L + (g->L & 1);
-    const size_t lds = (size_t)Le * 4 + (size_t)2 * K * nb * 4 + 3 * (size_t)nb * 8 + nw * 4 + 64;
+    const size_t base = (size_t)Le * 4 + (size_t)2 * K * nb * 4 + 4 * (size_t)nb * 8 + nw * 4 + 64;
+    int64_t me = 0;
+    for (int i0 = 0; i0 < g->n; i0 += nb)
+        me = std::max<int64_t>(me, g->rp_host[std::min(g->n, i0 + nb)] - g->rp_host[i0]);
+    const size_t cap = 160 * 1024 - 2048;
+    const int ne_lds = (base + (size_t)me * 12 <= cap) ? (int)std::max<int64_t>(me, 1) : 0;
+    const size_t lds = base + (size_t)ne_lds * 12;
+    a.ne_lds = ne_lds;
     void* args[] = {&a};
     SDEV_CK(hipFuncSetAttribute((const void*)k_fast, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     SDEV_CK(hipLaunchCooperativeKernel((const void*)k_fast, dim3(G), dim3(kFastThreads), args, (unsigned)lds,

@@ -28,6 +28,7 @@
 // more than one step ahead of another and two parity slots suffice.  Waits
 // are bounded: on timeout the launch aborts and reports an error.
 #include "dev_internal.hpp"
+#include "dev_exchange.hpp"
 
 #include <algorithm>
 #include <cstring>
@@ -36,7 +37,6 @@
 namespace sdev {
 
 constexpr int kPersistThreads = 128;
-constexpr long long kSpinLimit = 1ll << 22;   // ~seconds of polling before the launch aborts
 
 struct PersistArgs {
     int n, din, L, G, wpg, micro, stype;
@@ -55,40 +55,117 @@ struct PersistArgs {
     double* rec_V; uint8_t* rec_S;
 };
 
-// h += sum_{q<cnt} c[kb+q] * ring[(sb-q)*STRIDE + lane]   (descending slots, ascending lags)
-// Blocks of 16 lags in two register sets used alternately (no copies), so the
-// loads of the next block are in flight while the current block's dependent
-// adds run.
-template <int STRIDE>
-__device__ __forceinline__ void gl_load16(const double* __restrict__ ring, const double* __restrict__ c, int kb,
-                                          int sb, int lane, double* v, double* w) {
-    const double* rb = ring + (sb - 15) * STRIDE + lane;
-#pragma unroll
-    for (int j = 0; j < 16; ++j) { v[j] = rb[(15 - j) * STRIDE]; w[j] = c[kb + j]; }
+// FP64 multiply and add with their issue order pinned: each is one IEEE
+// round-to-nearest operation (the same as the compiler's v_mul_f64 /
+// v_add_f64), but the compiler may not move a product next to the add that
+// consumes it, which would expose the multiply latency on the add chain.
+__device__ __forceinline__ void add_f64_ordered(double& h, double p) {
+    asm volatile("v_add_f64 %0, %0, %1" : "+v"(h) : "v"(p));
 }
-__device__ __forceinline__ double gl_acc16(double h, const double* v, const double* w) {
-#pragma unroll
-    for (int j = 0; j < 16; ++j) h = h + w[j] * v[j];
-    return h;
+__device__ __forceinline__ void mul_f64_ordered(double& r, double a, double b) {
+    asm volatile("v_mul_f64 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
 }
-template <int STRIDE>
-__device__ __forceinline__ double gl_seg_lds(double h, const double* __restrict__ ring, const double* __restrict__ c,
-                                             int kb, int sb, int cnt, int lane) {
-    const int nblk = cnt >> 4;
-    int b = 0;
-    if (nblk > 0) {
-        double va[16], wa[16], vb[16], wb[16];
-        gl_load16<STRIDE>(ring, c, kb, sb, lane, va, wa);
-        while (b + 2 <= nblk) {
-            gl_load16<STRIDE>(ring, c, kb + 16 * (b + 1), sb - 16 * (b + 1), lane, vb, wb);
-            h = gl_acc16(h, va, wa);
-            if (b + 2 < nblk) gl_load16<STRIDE>(ring, c, kb + 16 * (b + 2), sb - 16 * (b + 2), lane, va, wa);
-            h = gl_acc16(h, vb, wb);
-            b += 2;
-        }
-        if (b < nblk) { h = gl_acc16(h, va, wa); ++b; }
+
+// One 16-lag block of a history row: 8 two-lag ring reads (.y lag q+2j,
+// .x lag q+2j+1) and the matching coefficient pairs.  sb - q is odd, so the
+// pair of lags (q+2j, q+2j+1) is the aligned slot pair (pb-j)*2, (pb-j)*2+1
+// with pb = (sb-q-1)/2: one base address per block, immediate offsets per pair.
+__device__ __forceinline__ void gl_block_load(const double2* __restrict__ row2, const double* __restrict__ cs, int kb,
+                                              int sb, int q, double2* v, double2* c) {
+    const double2* rb = row2 + ((sb - q - 1) >> 1) - 7;
+    const double* cb = cs + kb + q;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        v[j] = rb[7 - j];
+        c[j] = make_double2(cb[2 * j], cb[2 * j + 1]);
     }
-    for (int q = 16 * nblk; q < cnt; ++q) h = h + c[kb + q] * ring[(sb - q) * STRIDE + lane];
+}
+// The final, partial block of a segment: pair indices below zero (lags past
+// the segment) are clamped to slot pair 0 and their products are never added.
+__device__ __forceinline__ void gl_block_load_clamped(const double2* __restrict__ row2, const double* __restrict__ cs,
+                                                      int kb, int sb, int q, double2* v, double2* c) {
+    const int pb = (sb - q - 1) >> 1;
+    const double* cb = cs + kb + q;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        v[j] = row2[max(pb - j, 0)];
+        c[j] = make_double2(cb[2 * j], cb[2 * j + 1]);
+    }
+}
+__device__ __forceinline__ void gl_block_products(double* pr, const double2* v, const double2* c) {
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        mul_f64_ordered(pr[2 * j], c[j].x, v[j].y);
+        mul_f64_ordered(pr[2 * j + 1], c[j].y, v[j].x);
+    }
+}
+// h += pr (block b, in lag order) while the products of block b+1 replace them.
+__device__ __forceinline__ void gl_block_step(double& h, double* pr, const double2* v, const double2* c) {
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        add_f64_ordered(h, pr[2 * j]);
+        mul_f64_ordered(pr[2 * j], c[j].x, v[j].y);
+        add_f64_ordered(h, pr[2 * j + 1]);
+        mul_f64_ordered(pr[2 * j + 1], c[j].y, v[j].x);
+    }
+}
+
+// h += sum_{q<cnt} c[kb+q] * row[sb-q]  over one neuron's LDS history row
+// (lag-contiguous layout [NB][LP], LP even): slots descend as lags ascend, so
+// one 16-byte read returns two consecutive lags.  The sum keeps one dependent
+// add per lag in ascending lag order (flif_gl.c:66-72).  Blocks of 16 lags are
+// software-pipelined: the ring and coefficient reads of block b+1 are in
+// flight while block b is multiplied, and the products of block b+1 are
+// formed while the adds of block b run.  cs is the coefficient table in LDS,
+// zero-padded by 16 entries past L-1 for the clamped final block.
+__device__ __forceinline__ double gl_seg_row(double h, const double* __restrict__ row, const double* __restrict__ cs,
+                                             int kb, int sb, int cnt) {
+    if (cnt <= 0) return h;
+    int q = 0;
+    if ((sb & 1) == 0) {   // peel one lag so that pairs (s-1, s) start at odd s
+        h = h + cs[kb] * row[sb];
+        q = 1;
+    }
+    const double2* row2 = reinterpret_cast<const double2*>(row);
+    const int nblk = (cnt - q) >> 4;
+    double pr[16];
+    double2 vA[8], cA[8], vB[8], cB[8];
+    if (nblk > 0) {
+        gl_block_load(row2, cs, kb, sb, q, vA, cA);
+        if (nblk == 1) {
+            gl_block_products(pr, vA, cA);
+        } else {
+            gl_block_load(row2, cs, kb, sb, q + 16, vB, cB);
+            gl_block_products(pr, vA, cA);                      // pr: block 0, B: block 1
+            int b = 1;
+            for (; b + 2 <= nblk - 1; b += 2) {
+                gl_block_load(row2, cs, kb, sb, q + 16 * (b + 1), vA, cA);
+                gl_block_step(h, pr, vB, cB);                   // adds block b-1, products block b
+                gl_block_load(row2, cs, kb, sb, q + 16 * (b + 2), vB, cB);
+                gl_block_step(h, pr, vA, cA);                   // adds block b, products block b+1
+            }
+            if (b + 1 <= nblk - 1) {
+                gl_block_load(row2, cs, kb, sb, q + 16 * (b + 1), vA, cA);
+                gl_block_step(h, pr, vB, cB);
+                gl_block_step(h, pr, vA, cA);
+            } else {
+                gl_block_step(h, pr, vB, cB);
+            }
+        }
+        q += 16 * nblk;
+    }
+    const int last = cnt - q;                                    // lags of the partial block, 0..15
+    if (last > 0) gl_block_load_clamped(row2, cs, kb, sb, q, vA, cA);   // in flight under the adds below
+    if (nblk > 0) {
+#pragma unroll
+        for (int j = 0; j < 16; ++j) add_f64_ordered(h, pr[j]);
+    }
+    if (last > 0) {
+        gl_block_products(pr, vA, cA);
+#pragma unroll
+        for (int j = 0; j < 15; ++j)
+            if (j < last) add_f64_ordered(h, pr[j]);
+    }
     return h;
 }
 
@@ -108,24 +185,24 @@ __device__ __forceinline__ double gl_seg_hbm(double h, const double* __restrict_
     return h;
 }
 
-__device__ __forceinline__ unsigned long long ld_word(const unsigned long long* p) {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
 template <int NT, int NB, bool RL>
 __global__ void __launch_bounds__(kPersistThreads) k_persist(PersistArgs a) {
-    extern __shared__ double lds[];
+    extern __shared__ __align__(16) double lds[];   // 16-byte aligned: b128 ring-pair reads
     constexpr bool GL = (NT == NT_FLIF_GL);
-    const int g = blockIdx.x, tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    // wave index and abort flag are read through readfirstlane so the compiler
+    // treats them as wave-uniform (scalar branches, scalar loop counters).
+    const int g = blockIdx.x, tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int i0 = g * NB;
     const int nloc = min(NB, a.n - i0);
     const int L = a.L;
     const int wpg = a.wpg;
     const int nw = a.G * wpg;
     // ---- LDS layout
-    double* ring_s = lds;                                   // [L][NB]      (RL)
-    double* coef_s = ring_s + (RL ? (size_t)L * NB : 0);    // [L]          (GL)
-    double* rec_s = coef_s + (GL ? L : 0);                  // [2][NB]
+    const int LP = L + (L & 1) + 2;                         // padded row length (even)
+    double* ring_s = lds;                                   // [NB][LP]     (RL)
+    double* coef_s = ring_s + (RL ? (size_t)LP * NB : 0);   // [L + 16]     (GL; zero past L-1)
+    double* rec_s = coef_s + (GL ? L + 16 : 0);             // [2][NB]
     double* ext_s = rec_s + 2 * NB;                         // [2][NB]
     double* eval_s = ext_s + 2 * NB;                        // [ne_lds]
     unsigned* bits_s = (unsigned*)(eval_s + a.ne_lds);      // [nw]
@@ -133,11 +210,11 @@ __global__ void __launch_bounds__(kPersistThreads) k_persist(PersistArgs a) {
     __shared__ int abort_s;
     if (tid == 0) abort_s = 0;
     if (GL) {
-        for (int k = tid; k < L; k += kPersistThreads) coef_s[k] = a.coef[k];
+        for (int k = tid; k < L + 16; k += kPersistThreads) coef_s[k] = k < L ? a.coef[k] : 0.0;
         if (RL)
             for (int e = tid; e < L * NB; e += kPersistThreads) {
                 const int s = e / NB, j = e % NB;
-                ring_s[e] = (j < nloc) ? a.ring[(size_t)s * a.n + i0 + j] : 0.0;
+                ring_s[j * LP + s] = (j < nloc) ? a.ring[(size_t)s * a.n + i0 + j] : 0.0;
             }
     }
     const int64_t E0 = a.rp[i0];
@@ -145,9 +222,7 @@ __global__ void __launch_bounds__(kPersistThreads) k_persist(PersistArgs a) {
     const bool csr_lds = a.ne_lds > 0;
     if (csr_lds)
         for (int e = tid; e < ne; e += kPersistThreads) {
-            const int j = a.col[E0 + e];
-            const int gg = j / NB, r = j - gg * NB;
-            eidx_s[e] = ((unsigned)(gg * wpg + (r >> 5)) << 5) | (unsigned)(r & 31);
+            eidx_s[e] = encode_col(a.col[E0 + e], NB, wpg);
             eval_s[e] = a.val[E0 + e];
         }
     // ---- per-lane state: wave 0 lane = neuron, wave 1 lane = row puller
@@ -157,7 +232,7 @@ __global__ void __launch_bounds__(kPersistThreads) k_persist(PersistArgs a) {
     double tp = (own && GL) ? a.tprev[i] : 0.0;
     bool spk = own ? (a.S[i] != 0) : false;
     int rs = 0, re = 0;
-    constexpr int kMaxDinReg = 16;
+    constexpr int kMaxDinReg = 4;     // W_in rows / inputs held in registers (larger din reads memory)
     double win[kMaxDinReg];
     const bool win_reg = a.din <= kMaxDinReg;
 #pragma unroll
@@ -168,32 +243,45 @@ __global__ void __launch_bounds__(kPersistThreads) k_persist(PersistArgs a) {
         if (win_reg)
             for (int j = 0; j < a.din; ++j) win[j] = a.W_in[(size_t)i * a.din + j];
     }
-    const double* p = a.np;
+    // Model parameters in VGPRs (uniform values, but scalar registers are the
+    // scarce resource in this kernel: keeping them there forces spills).
+    double p[7];
+#pragma unroll
+    for (int j = 0; j < 7; ++j) {
+        p[j] = a.np[j];
+        asm volatile("" : "+v"(p[j]));
+    }
     __syncthreads();
 
     long long st = a.st0;
+    // ring head (st mod L) and history length min(st, L-1), advanced per micro-step
+    int head = GL ? (int)(st % L) : 0;
+    int lim = GL ? (st < L ? (int)st : L - 1) : 0;
     for (long long t = 0; t < a.T; ++t) {
         const long long tt = a.t0 + t;
         const int tb = (int)(t & 1);
         for (int mi = 0; mi < a.micro; ++mi, ++st) {
             const int sb = (int)(st & 1);
-            const int head = GL ? (int)(st % L) : 0;
             if (wave == 0) {
                 // ---------------- GL history (overlaps the exchange on wave 1)
+                // Every lane runs the sum in uniform control flow (lanes without
+                // a live neuron read a valid row and discard the result), so the
+                // coefficient loads stay scalar and the loop stays unmasked.
                 double H = 0.0;
-                if (GL && own && tp > p[6]) {
-                    const int lim = st < L ? (int)st : L - 1;
+                if (GL) {
                     const int n1 = lim < head ? lim : head;
                     if (RL) {
-                        H = gl_seg_lds<NB>(H, ring_s, coef_s, 1, head - 1, n1, lane);
-                        H = gl_seg_lds<NB>(H, ring_s, coef_s, n1 + 1, head - n1 - 1 + L, lim - n1, lane);
+                        const double* row = ring_s + (lane < NB ? lane : NB - 1) * LP;
+                        H = gl_seg_row(H, row, coef_s, 1, head - 1, n1);
+                        H = gl_seg_row(H, row, coef_s, n1 + 1, head - n1 - 1 + L, lim - n1);
                     } else {
-                        H = gl_seg_hbm(H, a.ring + i, coef_s, 1, head - 1, n1, (size_t)a.n);
-                        H = gl_seg_hbm(H, a.ring + i, coef_s, n1 + 1, head - n1 - 1 + L, lim - n1, (size_t)a.n);
+                        const double* col = a.ring + i0 + (lane < nloc ? lane : nloc - 1);
+                        H = gl_seg_hbm(H, col, coef_s, 1, head - 1, n1, (size_t)a.n);
+                        H = gl_seg_hbm(H, col, coef_s, n1 + 1, head - n1 - 1 + L, lim - n1, (size_t)a.n);
                     }
                 }
                 __syncthreads();   // rec / ext of this step ready
-                if (abort_s) break;
+                if (__builtin_amdgcn_readfirstlane(abort_s)) break;
                 int ring_slot_write = 0;
                 if (own) {
                     const double x = ext_s[tb * NB + lane] + rec_s[sb * NB + lane];
@@ -208,7 +296,7 @@ __global__ void __launch_bounds__(kPersistThreads) k_persist(PersistArgs a) {
                     } else {                                       // flif_gl.c:50-100
                         const int sp = head == 0 ? L - 1 : head - 1;
                         if (tp > p[6]) {
-                            const double Vp = RL ? ring_s[sp * NB + lane] : a.ring[(size_t)sp * a.n + i];
+                            const double Vp = RL ? ring_s[lane * LP + sp] : a.ring[(size_t)sp * a.n + i];
                             const double rhs = (-(Vp - p[2]) / p[3]) + x + p[5];
                             V = p[4] * rhs - H;
                             if (V >= p[0]) { V = p[1]; spk = true; tp = 0.0; }
@@ -218,17 +306,11 @@ __global__ void __launch_bounds__(kPersistThreads) k_persist(PersistArgs a) {
                         ring_slot_write = 1;
                     }
                 }
-                // publish first (the other workgroups wait on it): word = [tag | bits of 32 neurons]
-                const unsigned long long m = __ballot(own && spk);
-                if (lane < wpg) {
-                    const unsigned bits = (unsigned)(m >> (32 * lane));
-                    const unsigned long long word = ((unsigned long long)(unsigned)(st + 1) << 32) | bits;
-                    __hip_atomic_store(a.xch + (size_t)sb * nw + (size_t)g * wpg + lane, word, __ATOMIC_RELAXED,
-                                       __HIP_MEMORY_SCOPE_AGENT);
-                }
+                // publish first (the other workgroups wait on it)
+                publish_spikes(a.xch, nw, g, wpg, st, lane, __ballot(own && spk));
                 if (own) {
                     if (GL && ring_slot_write) {
-                        if (RL) ring_s[head * NB + lane] = V;
+                        if (RL) ring_s[lane * LP + head] = V;
                         else a.ring[(size_t)head * a.n + i] = V;
                         tp = tp + a.dt;
                     }
@@ -245,106 +327,15 @@ __global__ void __launch_bounds__(kPersistThreads) k_persist(PersistArgs a) {
                     for (int j = 0; j < kMaxDinReg; ++j) uin[j] = (j < a.din) ? a.u[tt * a.din + j] : 0.0;
                 }
                 // ---------------- spikes of micro-step st-1
-                bool aborted = false;
-                if (st == a.st0) {
-                    for (int w = lane; w < nw; w += 64) {   // left in HBM by the previous launch
-                        const int gg = w / wpg, part = w % wpg;
-                        unsigned b = 0;
-                        for (int j = 0; j < 32; ++j) {
-                            const int nj = gg * NB + part * 32 + j;
-                            if (part * 32 + j < NB && nj < a.n && a.S[nj]) b |= 1u << j;
-                        }
-                        bits_s[w] = b;
-                    }
-                } else {
-                    const unsigned long long* slot = a.xch + (size_t)(sb ^ 1) * nw;
-                    const unsigned tag = (unsigned)st;       // tag of micro-step st-1
-                    constexpr int kMaxWordsPerLane = 8;      // nw <= 512
-                    unsigned long long word[kMaxWordsPerLane];
-                    bool ready[kMaxWordsPerLane];
-#pragma unroll
-                    for (int q = 0; q < kMaxWordsPerLane; ++q) { ready[q] = lane + 64 * q >= nw; word[q] = 0; }
-                    long long spins = 0;
-                    while (true) {
-#pragma unroll
-                        for (int q = 0; q < kMaxWordsPerLane; ++q)
-                            if (!ready[q]) word[q] = ld_word(&slot[lane + 64 * q]);
-                        bool all_here = true;
-#pragma unroll
-                        for (int q = 0; q < kMaxWordsPerLane; ++q) {
-                            if (!ready[q]) ready[q] = (unsigned)(word[q] >> 32) == tag;
-                            all_here = all_here && ready[q];
-                        }
-                        if (__all(all_here)) break;
-                        const bool give_up = (++spins > kSpinLimit) ||
-                            __hip_atomic_load(a.abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u;
-                        if (__any(give_up)) {
-                            aborted = true;
-                            if (lane == 0) {
-                                abort_s = 1;
-                                __hip_atomic_store(a.abort_flag, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                            }
-                            break;
-                        }
-                        __builtin_amdgcn_s_sleep(1);
-                    }
-#pragma unroll
-                    for (int q = 0; q < kMaxWordsPerLane; ++q)
-                        if (lane + 64 * q < nw) bits_s[lane + 64 * q] = (unsigned)word[q];
-                }
+                const bool aborted = !gather_spikes(a.xch, nw, NB, wpg, a.n, st, a.st0, a.S, bits_s,
+                                                    a.abort_flag, &abort_s, lane);
                 __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
                 __builtin_amdgcn_wave_barrier();
                 __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
                 // ---------------- CSR row pull and external input
                 if (!aborted && lane < nloc) {
-                    double s = 0.0;
-                    if (a.stype == ST_SIMPLE) {
-                        if (csr_lds) {
-                            // batches of 8 edges: all LDS loads issued, then the in-order sum
-                            int e = rs;
-                            for (; e + 8 <= re; e += 8) {
-                                unsigned pk[8], bw[8];
-                                double wv[8];
-#pragma unroll
-                                for (int q = 0; q < 8; ++q) { pk[q] = eidx_s[e + q]; wv[q] = eval_s[e + q]; }
-#pragma unroll
-                                for (int q = 0; q < 8; ++q) bw[q] = bits_s[pk[q] >> 5];
-#pragma unroll
-                                for (int q = 0; q < 8; ++q)
-                                    if ((bw[q] >> (pk[q] & 31)) & 1u) s = s + wv[q];
-                            }
-                            for (; e < re; ++e) {
-                                const unsigned pk = eidx_s[e];
-                                if ((bits_s[pk >> 5] >> (pk & 31)) & 1u) s = s + eval_s[e];
-                            }
-                        } else {
-                            for (int64_t k = E0 + rs; k < E0 + re; ++k) {
-                                const int j = a.col[k];
-                                const int gg = j / NB, r = j - gg * NB;
-                                if ((bits_s[gg * wpg + (r >> 5)] >> (r & 31)) & 1u) s = s + a.val[k];
-                            }
-                        }
-                    } else {   // PSC traces per edge (homogeneous: uniform decay)
-                        for (int e = rs; e < re; ++e) {
-                            const int64_t k = E0 + e;
-                            unsigned word_i, bit;
-                            double w;
-                            if (csr_lds) {
-                                const unsigned pk = eidx_s[e];
-                                word_i = pk >> 5; bit = pk & 31; w = eval_s[e];
-                            } else {
-                                const int j = a.col[k];
-                                const int gg = j / NB, r = j - gg * NB;
-                                word_i = (unsigned)(gg * wpg + (r >> 5)); bit = (unsigned)(r & 31); w = a.val[k];
-                            }
-                            const double sj = (double)((bits_s[word_i] >> bit) & 1u);
-                            const double d = a.edge_decay[k];
-                            const double tr = a.edge_tr[k] * d + (1.0 - d) * sj;
-                            a.edge_tr[k] = tr;
-                            s = s + w * tr;
-                        }
-                    }
-                    rec_s[sb * NB + lane] = s;
+                    rec_s[sb * NB + lane] = row_pull(a.stype, csr_lds, rs, re, E0, eidx_s, eval_s, a.col, a.val,
+                                                     bits_s, NB, wpg, a.edge_decay, a.edge_tr);
                     if (mi == 0) {
                         double su = 0.0;
                         if (win_reg) {
@@ -358,10 +349,14 @@ __global__ void __launch_bounds__(kPersistThreads) k_persist(PersistArgs a) {
                     }
                 }
                 __syncthreads();   // rec / ext of this step ready
-                if (abort_s) break;
+                if (__builtin_amdgcn_readfirstlane(abort_s)) break;
+            }
+            if (GL) {
+                head = head + 1 == L ? 0 : head + 1;
+                if (lim < L - 1) ++lim;
             }
         }
-        if (abort_s) break;
+        if (__builtin_amdgcn_readfirstlane(abort_s)) break;
     }
     // write back the persistent state
     if (own) {
@@ -373,8 +368,304 @@ __global__ void __launch_bounds__(kPersistThreads) k_persist(PersistArgs a) {
         __syncthreads();
         for (int e = tid; e < L * NB; e += kPersistThreads) {
             const int s = e / NB, j = e % NB;
-            if (j < nloc) a.ring[(size_t)s * a.n + i0 + j] = ring_s[e];
+            if (j < nloc) a.ring[(size_t)s * a.n + i0 + j] = ring_s[j * LP + s];
         }
+    }
+}
+
+
+// ---------------------------------------------------------------------------
+// FLIF-GL persistent kernel with a trailing product stage.
+//
+// The GL history of micro-step m is H_m = sum_{k=1..lim} c_k V_{m-k}, summed
+// in ascending k with one dependent add per lag (flif_gl.c:66-72).  The
+// products are independent of each other, so they are formed off the chain:
+//   wave 0        (chain)     H = c_1 V_{m-1}, then adds P_m[2..lim] from LDS in
+//                             lag order, then the neuron tail (update, publish,
+//                             history write);
+//   waves 1,2,3,5 (products)  during step m write P_{m+1}[k] = c_k V_{m+1-k}
+//                             into the same LDS rows, each 32-lag chunk only
+//                             after the chain of step m has consumed it;
+//   wave 4        (exchange)  spike exchange, CSR row pull and external input.
+// The products of step m+1 need V_{m-1} (lag 2; handed over through a
+// two-slot LDS stage by the tail of step m-1) and V_{m-2} and older (read from
+// the history ring in HBM, [L][N], written by the tail two or more steps
+// earlier and ordered by the workgroup barrier).  The chain wave never waits
+// on the producers except at the one barrier per step; the producers follow
+// its progress through an LDS counter.  Products and sums are plain IEEE
+// double operations in the reference's order, so the result is bit-identical
+// to the single-wave sum.
+constexpr int kTpThreads = 384;
+constexpr int kTpChunk = 32;
+constexpr int kTpProducers = 4;
+constexpr long long kTpSpinLimit = 1ll << 24;
+
+// Chain progress: 64 * (micro-step in launch) + chunks done; compared modulo 2^32.
+__device__ __forceinline__ unsigned lds_counter_load(const unsigned* f) {
+    return __hip_atomic_load(f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+__device__ __forceinline__ void lds_counter_store(unsigned* f, unsigned v) {
+    __hip_atomic_store(f, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+// Product-row stride in doubles: every chunk of 32 lags starting at lag 2 fits,
+// and 2*PS = 4 (mod 32) dwords keeps the rows' 16-byte reads conflict-free.
+__host__ __device__ __forceinline__ int tp_row_stride(int L) { return 32 * ((L - 2 + 31) / 32) + 2; }
+
+template <int NB>
+__global__ void __launch_bounds__(kTpThreads) k_persist_tp(PersistArgs a) {
+    constexpr int C = kTpChunk;
+    constexpr int PARTS = 64 * kTpProducers / NB;   // producer lanes per neuron
+    constexpr int PER = C / PARTS;                  // lags per producer lane per chunk
+    static_assert(PER >= 2 && PER % 2 == 0, "producer lanes write lag pairs");
+    extern __shared__ __align__(16) double lds[];   // 16-byte aligned: b128 product reads
+    const int g = blockIdx.x, tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int i0 = g * NB;
+    const int nloc = min(NB, a.n - i0);
+    const int n = a.n, L = a.L, wpg = a.wpg, nw = a.G * wpg;
+    const int PS = tp_row_stride(L);
+    // ---- LDS layout
+    double* P = lds;                                   // [NB][PS]   P[r][k]
+    double* coef_s = P + (size_t)NB * PS;              // [PS]       zero past L-1
+    double* stage = coef_s + PS;                       // [2][NB]    V of the last two steps
+    double* rec_s = stage + 2 * NB;                    // [2][NB]
+    double* ext_s = rec_s + 2 * NB;                    // [2][NB]
+    double* eval_s = ext_s + 2 * NB;                   // [ne_lds]
+    unsigned* bits_s = (unsigned*)(eval_s + a.ne_lds); // [nw]
+    unsigned* eidx_s = bits_s + nw;                    // [ne_lds]
+    __shared__ int abort_s;
+    __shared__ unsigned cons_s;
+    if (tid == 0) { abort_s = 0; cons_s = 0u; }
+    for (int k = tid; k < PS; k += kTpThreads) coef_s[k] = k < L ? a.coef[k] : 0.0;
+    const int head0 = (int)(a.st0 % L);
+    {
+        const int sp = head0 == 0 ? L - 1 : head0 - 1;   // V_{st0-1}
+        const int slot = (int)((a.st0 - 1) & 1);
+        for (int e = tid; e < NB; e += kTpThreads)
+            stage[slot * NB + e] = e < nloc ? a.ring[(size_t)sp * n + i0 + e] : 0.0;
+    }
+    const int64_t E0 = a.rp[i0];
+    const int ne = (int)(a.rp[i0 + nloc] - E0);
+    const bool csr_lds = a.ne_lds > 0;
+    if (csr_lds)
+        for (int e = tid; e < ne; e += kTpThreads) {
+            eidx_s[e] = encode_col(a.col[E0 + e], NB, wpg);
+            eval_s[e] = a.val[E0 + e];
+        }
+    const int r = lane % NB;                           // neuron row of this lane (chain, producers)
+    const int ir = i0 + (r < nloc ? r : nloc - 1);     // clamped global index for reads
+    const bool own = (wave == 0) && (lane < nloc);
+    const int i = i0 + lane;
+    double V = own ? a.V[i] : 0.0;
+    double tp = own ? a.tprev[i] : 0.0;
+    bool spk = own ? (a.S[i] != 0) : false;
+    int rs = 0, re = 0;
+    constexpr int kMaxDinReg = 4;
+    double win[kMaxDinReg];
+    const bool win_reg = a.din <= kMaxDinReg;
+#pragma unroll
+    for (int j = 0; j < kMaxDinReg; ++j) win[j] = 0.0;
+    if (wave == 4 && lane < nloc) {
+        rs = (int)(a.rp[i] - E0);
+        re = (int)(a.rp[i + 1] - E0);
+        if (win_reg)
+            for (int j = 0; j < a.din; ++j) win[j] = a.W_in[(size_t)i * a.din + j];
+    }
+    double p[7];
+#pragma unroll
+    for (int j = 0; j < 7; ++j) {
+        p[j] = a.np[j];
+        asm volatile("" : "+v"(p[j]));
+    }
+    const bool producer = wave == 1 || wave == 2 || wave == 3 || wave == 5;
+    const int pw = wave == 5 ? 3 : wave - 1;
+    const int part = pw * (64 / NB) + lane / NB;
+    int head = head0;
+    int lim = a.st0 < L ? (int)a.st0 : L - 1;
+    // products of the launch's first step, all from the HBM ring
+    if (producer) {
+        const int nk = lim >= 2 ? lim - 1 : 0;
+        for (int k0 = 2 + part * PER; k0 < 2 + nk; k0 += C) {
+#pragma unroll
+            for (int j = 0; j < PER; j += 2) {
+                const int k = k0 + j;
+                const int ka = k < L ? k : L - 1, kb = k + 1 < L ? k + 1 : L - 1;
+                int sa = head - ka; if (sa < 0) sa += L;
+                int sbb = head - kb; if (sbb < 0) sbb += L;
+                *reinterpret_cast<double2*>(P + (size_t)r * PS + k) =
+                    make_double2(coef_s[k] * a.ring[(size_t)sa * n + ir], coef_s[k + 1] * a.ring[(size_t)sbb * n + ir]);
+            }
+        }
+    }
+    __syncthreads();
+
+    const double c1 = coef_s[1];
+    const double2* prow = reinterpret_cast<const double2*>(P + (size_t)r * PS);   // pair j: lags 2j, 2j+1
+    long long st = a.st0;
+    unsigned m = 0;                                    // micro-step index within the launch
+    for (long long t = 0; t < a.T; ++t) {
+        const long long tt = a.t0 + t;
+        const int tb = (int)(t & 1);
+        for (int mi = 0; mi < a.micro; ++mi, ++st, ++m) {
+            const int sb = (int)(st & 1);
+            const int nk = lim >= 2 ? lim - 1 : 0;     // lags 2..lim of this step
+            const int nch = (nk + C - 1) / C;
+            if (wave == 0) {
+                // ---------------- chain: H = c_1 V_{st-1} + sum_k P[k]
+                const double Vprev = stage[(int)((st - 1) & 1) * NB + r];
+                double h = 0.0;
+                if (lim >= 1) h = h + c1 * Vprev;
+                double2 bA[C / 2], bB[C / 2];
+                auto ld = [&](double2* buf, int c) {
+#pragma unroll
+                    for (int j = 0; j < C / 2; ++j) buf[j] = prow[1 + (C / 2) * c + j];
+                };
+                auto add = [&](const double2* buf, int cnt) {
+                    if (cnt == C) {
+#pragma unroll
+                        for (int j = 0; j < C / 2; ++j) { add_f64_ordered(h, buf[j].x); add_f64_ordered(h, buf[j].y); }
+                    } else {
+#pragma unroll
+                        for (int j = 0; j < C / 2; ++j) {
+                            if (2 * j < cnt) add_f64_ordered(h, buf[j].x);
+                            if (2 * j + 1 < cnt) add_f64_ordered(h, buf[j].y);
+                        }
+                    }
+                };
+                if (nch > 0) ld(bA, 0);
+                int c = 0;
+                for (; c + 2 <= nch; c += 2) {
+                    ld(bB, c + 1);
+                    add(bA, C);
+                    lds_counter_store(&cons_s, m * 64u + (unsigned)c + 1u);
+                    if (c + 2 < nch) ld(bA, c + 2);
+                    add(bB, (c + 1 == nch - 1) ? nk - (c + 1) * C : C);
+                    lds_counter_store(&cons_s, m * 64u + (unsigned)c + 2u);
+                }
+                if (c < nch) {
+                    add(bA, nk - c * C);
+                    lds_counter_store(&cons_s, m * 64u + (unsigned)c + 1u);
+                }
+                __syncthreads();   // rec / ext of this step and the products of the next are ready
+                if (__builtin_amdgcn_readfirstlane(abort_s)) break;
+                if (own) {                                  // flif_gl.c:50-100
+                    const double x = ext_s[tb * NB + lane] + rec_s[sb * NB + lane];
+                    spk = false;
+                    if (tp > p[6]) {
+                        const double rhs = (-(Vprev - p[2]) / p[3]) + x + p[5];
+                        V = p[4] * rhs - h;
+                        if (V >= p[0]) { V = p[1]; spk = true; tp = 0.0; }
+                    } else {
+                        V = p[1];
+                    }
+                }
+                publish_spikes(a.xch, nw, g, wpg, st, lane, __ballot(own && spk));
+                if (own) {
+                    a.ring[(size_t)head * n + i] = V;
+                    stage[sb * NB + lane] = V;
+                    tp = tp + a.dt;
+                    if (mi == a.micro - 1) {
+                        if (a.rec_V) a.rec_V[(size_t)t * n + i] = V;
+                        if (a.rec_S) a.rec_S[(size_t)t * n + i] = spk ? 1 : 0;
+                    }
+                }
+                lds_counter_store(&cons_s, (m + 1u) * 64u);   // tail done: V_st staged
+            } else if (wave == 4) {
+                // ---------------- exchange, row pull, external input
+                double uin[kMaxDinReg];
+                if (mi == 0 && win_reg) {
+#pragma unroll
+                    for (int j = 0; j < kMaxDinReg; ++j) uin[j] = (j < a.din) ? a.u[tt * a.din + j] : 0.0;
+                }
+                const bool aborted = !gather_spikes(a.xch, nw, NB, wpg, n, st, a.st0, a.S, bits_s,
+                                                    a.abort_flag, &abort_s, lane);
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                if (!aborted && lane < nloc) {
+                    rec_s[sb * NB + lane] = row_pull(a.stype, csr_lds, rs, re, E0, eidx_s, eval_s, a.col, a.val,
+                                                     bits_s, NB, wpg, a.edge_decay, a.edge_tr);
+                    if (mi == 0) {
+                        double su = 0.0;
+                        if (win_reg) {
+#pragma unroll
+                            for (int j = 0; j < kMaxDinReg; ++j)
+                                if (j < a.din) su = su + win[j] * uin[j];
+                        } else {
+                            for (int j = 0; j < a.din; ++j) su = su + a.W_in[(size_t)i * a.din + j] * a.u[tt * a.din + j];
+                        }
+                        ext_s[tb * NB + lane] = a.gain * su;
+                    }
+                }
+                __syncthreads();
+                if (__builtin_amdgcn_readfirstlane(abort_s)) break;
+            } else {
+                // ---------------- products of step st+1, behind the chain of step st
+                const int headn = head + 1 == L ? 0 : head + 1;
+                const int limn = lim < L - 1 ? lim + 1 : L - 1;
+                const int nkn = limn >= 2 ? limn - 1 : 0;
+                const int nchn = (nkn + C - 1) / C;
+                auto ldr = [&](double* v, int c) {         // V_{st+1-k}, k >= 3: final since the last barrier
+#pragma unroll
+                    for (int j = 0; j < PER; ++j) {
+                        const int k = 2 + c * C + part * PER + j;
+                        const int kk = k < L - 1 ? (k > 3 ? k : 3) : L - 1;
+                        int s = headn - kk; if (s < 0) s += L;
+                        v[j] = a.ring[(size_t)s * n + ir];
+                    }
+                };
+                bool stalled = false;
+                auto put = [&](const double* v, int c) {
+                    double2 pr[PER / 2];
+#pragma unroll
+                    for (int j = 0; j < PER / 2; ++j) {
+                        const int k = 2 + c * C + part * PER + 2 * j;
+                        pr[j] = make_double2(coef_s[k] * v[2 * j], coef_s[k + 1] * v[2 * j + 1]);
+                    }
+                    // the chain of step st is past chunk c (or does not read it) and its tail
+                    // of step st-1 has staged V_{st-1}
+                    const unsigned target = m * 64u + (unsigned)(c + 1 < nch ? c + 1 : nch);
+                    long long spins = 0;
+                    while ((int)(lds_counter_load(&cons_s) - target) < 0 && !stalled) {
+                        if (++spins > kTpSpinLimit) {
+                            stalled = true;
+                            __hip_atomic_store(&abort_s, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                            __hip_atomic_store(a.abort_flag, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        }
+                        __builtin_amdgcn_s_sleep(1);
+                    }
+                    if (c == 0 && part == 0)
+                        pr[0].x = coef_s[2] * stage[(int)((st - 1) & 1) * NB + r];   // lag 2: V_{st-1}
+                    double2* dst = reinterpret_cast<double2*>(P + (size_t)r * PS + 2 + c * C + part * PER);
+#pragma unroll
+                    for (int j = 0; j < PER / 2; ++j) dst[j] = pr[j];
+                };
+                double vA[PER], vB[PER], vC[PER];
+                if (nchn > 0) ldr(vA, 0);
+                if (nchn > 1) ldr(vB, 1);
+                int c = 0;
+                for (; c + 3 <= nchn; c += 3) {
+                    ldr(vC, c + 2);
+                    put(vA, c);
+                    if (c + 3 < nchn) ldr(vA, c + 3);
+                    put(vB, c + 1);
+                    if (c + 4 < nchn) ldr(vB, c + 4);
+                    put(vC, c + 2);
+                }
+                if (c < nchn) put(vA, c);
+                if (c + 1 < nchn) put(vB, c + 1);
+                __syncthreads();
+                if (__builtin_amdgcn_readfirstlane(abort_s)) break;
+            }
+            head = head + 1 == L ? 0 : head + 1;
+            if (lim < L - 1) ++lim;
+        }
+        if (__builtin_amdgcn_readfirstlane(abort_s)) break;
+    }
+    if (own) {
+        a.V[i] = V;
+        a.S_out[i] = spk ? 1 : 0;
+        a.tprev[i] = tp;
     }
 }
 
@@ -382,6 +673,7 @@ namespace {
 
 struct Plan {
     int nb = 0, G = 0, ring_lds = 0, ne_lds = 0;
+    int tp = 0;                 // FLIF-GL with the trailing product stage (k_persist_tp)
     size_t lds = 0;
 };
 
@@ -402,7 +694,8 @@ bool persist_plan(spires_dev* g, int ncu, Plan* pl) {
         const int G = (n + nb - 1) / nb;
         const int wpg = (nb + 31) / 32;
         if (G > ncu || G * wpg > 512) return false;
-        size_t base = (rl ? (size_t)L * nb * 8 : 0) + (gl ? (size_t)L * 8 : 0) + 4 * (size_t)nb * 8 +
+        const size_t LP = (size_t)L + (L & 1) + 2;
+        size_t base = (rl ? LP * nb * 8 : 0) + (gl ? (size_t)(L + 16) * 8 : 0) + 4 * (size_t)nb * 8 +
                       (size_t)G * wpg * 4 + 64;
         if (base > cap) return false;
         const int64_t me = max_edges(nb);
@@ -411,9 +704,29 @@ bool persist_plan(spires_dev* g, int ncu, Plan* pl) {
         pl->lds = base + (size_t)pl->ne_lds * 12;
         return true;
     };
+    auto fill_tp = [&](int nb) -> bool {
+        const int G = (n + nb - 1) / nb;
+        const int wpg = (nb + 31) / 32;
+        if (L < 2 || G > ncu || G * wpg > 512) return false;
+        const size_t PS = (size_t)tp_row_stride(L);
+        const size_t base = PS * nb * 8 + PS * 8 + 6 * (size_t)nb * 8 + (size_t)G * wpg * 4 + 64;
+        if (base > cap) return false;
+        const int64_t me = max_edges(nb);
+        pl->ne_lds = (base + (size_t)me * 12 <= cap) ? (int)std::max<int64_t>(me, 1) : 0;
+        pl->nb = nb; pl->G = G; pl->ring_lds = 0; pl->tp = 1;
+        pl->lds = base + (size_t)pl->ne_lds * 12;
+        return true;
+    };
     if (gl) {
-        for (int nb : {32, 16})
-            if (fill(nb, 1)) return true;
+        // path 2 / auto: trailing-product kernel, then the single-wave chain with the
+        // history in LDS, then in HBM; path 3 / 4 force the single-wave HBM / LDS variants.
+        if (g->path == 0 || g->path == 2)
+            for (int nb : {32, 16})
+                if (fill_tp(nb)) return true;
+        if (g->path != 3)
+            for (int nb : {32, 16})
+                if (fill(nb, 1)) return true;
+        if (g->path == 4) return false;
         for (int nb : {32, 64})
             if (fill(nb, 0)) return true;
         return false;
@@ -488,13 +801,19 @@ int launch_persist(spires_dev* g, const double* u_series, long long t0, long lon
     a.rec_V = rec_V; a.rec_S = rec_S;
     void* args[] = {&a};
     const void* fn = nullptr;
-    switch (g->ntype) {
-    case NT_LIF_DISCRETE: fn = pick_kernel<NT_LIF_DISCRETE>(pl.nb, pl.ring_lds); break;
-    case NT_LIF_BIO: fn = pick_kernel<NT_LIF_BIO>(pl.nb, pl.ring_lds); break;
-    default: fn = pick_kernel<NT_FLIF_GL>(pl.nb, pl.ring_lds); break;
+    int threads = kPersistThreads;
+    if (pl.tp) {
+        fn = pl.nb == 32 ? (const void*)k_persist_tp<32> : (const void*)k_persist_tp<16>;
+        threads = kTpThreads;
+    } else {
+        switch (g->ntype) {
+        case NT_LIF_DISCRETE: fn = pick_kernel<NT_LIF_DISCRETE>(pl.nb, pl.ring_lds); break;
+        case NT_LIF_BIO: fn = pick_kernel<NT_LIF_BIO>(pl.nb, pl.ring_lds); break;
+        default: fn = pick_kernel<NT_FLIF_GL>(pl.nb, pl.ring_lds); break;
+        }
     }
     SDEV_CK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)pl.lds));
-    SDEV_CK(hipLaunchCooperativeKernel(fn, dim3(pl.G), dim3(kPersistThreads), args, (unsigned)pl.lds, g->stream));
+    SDEV_CK(hipLaunchCooperativeKernel(fn, dim3(pl.G), dim3(threads), args, (unsigned)pl.lds, g->stream));
     g->step += (long long)T * g->micro;
     g->msteps += (size_t)T;
     g->cur ^= 1;

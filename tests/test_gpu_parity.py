@@ -215,12 +215,14 @@ def test_invalid_inputs_raise_cleanly():
     assert L.spires_train_ridge(r.h, None, None, 5, 1.0) == 1
 
 
-@pytest.mark.parametrize("path", [sa.PATH_PER_STEP, sa.PATH_PERSISTENT])
+@pytest.mark.parametrize("path", [sa.PATH_PER_STEP, sa.PATH_PERSISTENT, sa.PATH_PERSISTENT_HBM,
+                                  sa.PATH_PERSISTENT_LDS])
 @pytest.mark.parametrize("case", [CASES[0], CASES[2], CASES[5], CASES[7], CASES[11]],
                          ids=["gl-ws", "gl-ba", "gl-psch", "gl-psche", "lifd"])
 def test_kernel_paths_agree_with_oracle(case, path):
-    """Both exact-mode kernel paths (one launch per micro-step, one persistent
-    launch per run) reproduce the oracle, including across several runs that
+    """Every exact-mode kernel path (one launch per micro-step; one persistent
+    launch per run with product waves feeding the history sum, or a single-wave
+    sum over the history in HBM or LDS) reproduces the oracle, including across several runs that
     continue the same state."""
     print("case", case, "path", path, flush=True)
     r, o = _pair(case, n=300)
