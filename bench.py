@@ -14,7 +14,11 @@ The CPU baseline is the oracle's C restatement of the reference step
 (oracle/liboracle.so, OpenMP over neurons) on a bounded sample of the same
 workload, timed on this host.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+With --copy-calibration a plain device-to-device copy of 256 MiB runs first:
+its FETCH_SIZE / WRITE_SIZE under rocprofv3 --pmc calibrate the counters that
+give the step kernel's HBM traffic.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--copy-calibration]
 """
 import argparse
 import ctypes as C
@@ -96,6 +100,8 @@ def main():
     ap.add_argument("--steps", type=int, default=20000)
     ap.add_argument("--warmup", type=int, default=200)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="first copy 256 MiB device-to-device (counter calibration for rocprofv3 --pmc)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs one reservoir per device and has no "
@@ -108,6 +114,15 @@ def main():
 
     if not os.path.exists(sa.LIB_PATH):
         sa.build()
+    if args.copy_calibration:
+        import torch
+        nbytes = 256 << 20
+        src = torch.ones(nbytes // 8, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes": nbytes}), flush=True)
+        del src, dst
     K, W = args.steps, args.warmup
     cfg, csr, W_in, u = workloads.config1(T=K + W)
     n = cfg[0].num_neurons

@@ -532,27 +532,30 @@ __global__ void __launch_bounds__(kTpThreads) k_persist_tp(PersistArgs a) {
                         }
                     }
                 };
-                if (nch > 0) ld(bA, 0);
-                int c = 0;
-                for (; c + 2 <= nch; c += 2) {
-                    ld(bB, c + 1);
-                    add(bA, C);
-                    lds_counter_store(&cons_s, m * 64u + (unsigned)c + 1u);
-                    if (c + 2 < nch) ld(bA, c + 2);
-                    add(bB, (c + 1 == nch - 1) ? nk - (c + 1) * C : C);
-                    lds_counter_store(&cons_s, m * 64u + (unsigned)c + 2u);
+                if (lane < NB) {   // lanes past NB would only repeat the rows of lanes 0..NB-1
+                    if (nch > 0) ld(bA, 0);
+                    int c = 0;
+                    for (; c + 2 <= nch; c += 2) {
+                        ld(bB, c + 1);
+                        add(bA, C);
+                        lds_counter_store(&cons_s, m * 64u + (unsigned)c + 1u);
+                        if (c + 2 < nch) ld(bA, c + 2);
+                        add(bB, (c + 1 == nch - 1) ? nk - (c + 1) * C : C);
+                        lds_counter_store(&cons_s, m * 64u + (unsigned)c + 2u);
+                    }
+                    if (c < nch) {
+                        add(bA, nk - c * C);
+                        lds_counter_store(&cons_s, m * 64u + (unsigned)c + 1u);
+                    }
                 }
-                if (c < nch) {
-                    add(bA, nk - c * C);
-                    lds_counter_store(&cons_s, m * 64u + (unsigned)c + 1u);
-                }
+                const double leak = -(Vprev - p[2]) / p[3];   // before the barrier: the wait hides it
                 __syncthreads();   // rec / ext of this step and the products of the next are ready
                 if (__builtin_amdgcn_readfirstlane(abort_s)) break;
                 if (own) {                                  // flif_gl.c:50-100
                     const double x = ext_s[tb * NB + lane] + rec_s[sb * NB + lane];
                     spk = false;
                     if (tp > p[6]) {
-                        const double rhs = (-(Vprev - p[2]) / p[3]) + x + p[5];
+                        const double rhs = leak + x + p[5];
                         V = p[4] * rhs - h;
                         if (V >= p[0]) { V = p[1]; spk = true; tp = 0.0; }
                     } else {
@@ -605,13 +608,19 @@ __global__ void __launch_bounds__(kTpThreads) k_persist_tp(PersistArgs a) {
                 const int limn = lim < L - 1 ? lim + 1 : L - 1;
                 const int nkn = limn >= 2 ? limn - 1 : 0;
                 const int nchn = (nkn + C - 1) / C;
-                auto ldr = [&](double* v, int c) {         // V_{st+1-k}, k >= 3: final since the last barrier
+                // V_{st+1-k}, k >= 3, is final since the last barrier: loads run three chunks
+                // ahead of the stores.  The slot of lag k = kbase + off is
+                // (headn - kbase - off) mod L with off < 2L; lag 2 (replaced by the staged
+                // value) and lags past lim are read but discarded.
+                const int sbase = headn - (2 + part * PER);
+                const double* ring_r = a.ring + ir;
+                auto ldr = [&](double* v, int c) {
 #pragma unroll
                     for (int j = 0; j < PER; ++j) {
-                        const int k = 2 + c * C + part * PER + j;
-                        const int kk = k < L - 1 ? (k > 3 ? k : 3) : L - 1;
-                        int s = headn - kk; if (s < 0) s += L;
-                        v[j] = a.ring[(size_t)s * n + ir];
+                        int s = sbase - (c * C + j);
+                        s += s < 0 ? L : 0;
+                        s += s < 0 ? L : 0;
+                        v[j] = ring_r[(size_t)s * n];
                     }
                 };
                 bool stalled = false;
@@ -640,20 +649,24 @@ __global__ void __launch_bounds__(kTpThreads) k_persist_tp(PersistArgs a) {
 #pragma unroll
                     for (int j = 0; j < PER / 2; ++j) dst[j] = pr[j];
                 };
-                double vA[PER], vB[PER], vC[PER];
+                double vA[PER], vB[PER], vC[PER], vD[PER];
                 if (nchn > 0) ldr(vA, 0);
                 if (nchn > 1) ldr(vB, 1);
+                if (nchn > 2) ldr(vC, 2);
                 int c = 0;
-                for (; c + 3 <= nchn; c += 3) {
-                    ldr(vC, c + 2);
+                for (; c + 4 <= nchn; c += 4) {
+                    ldr(vD, c + 3);
                     put(vA, c);
-                    if (c + 3 < nchn) ldr(vA, c + 3);
+                    if (c + 4 < nchn) ldr(vA, c + 4);
                     put(vB, c + 1);
-                    if (c + 4 < nchn) ldr(vB, c + 4);
+                    if (c + 5 < nchn) ldr(vB, c + 5);
                     put(vC, c + 2);
+                    if (c + 6 < nchn) ldr(vC, c + 6);
+                    put(vD, c + 3);
                 }
                 if (c < nchn) put(vA, c);
                 if (c + 1 < nchn) put(vB, c + 1);
+                if (c + 2 < nchn) put(vC, c + 2);
                 __syncthreads();
                 if (__builtin_amdgcn_readfirstlane(abort_s)) break;
             }
