@@ -67,6 +67,17 @@ def algorithmic_bytes_per_neuron_step(n, nnz, L):
     return 8 * (L - 1) + 8 + (nnz / n) * 12 + 8 + 8 + 1 + 8
 
 
+def measured_traffic_per_step():
+    """HBM bytes per step of the step kernel from the committed rocprofv3 --pmc
+    summary (FETCH_SIZE / WRITE_SIZE, copy-calibrated) for this workload."""
+    f = os.path.join(ROOT, "profiles", "k_persist_tp_flifgl_n4096_L512.json")
+    if not os.path.exists(f):
+        return None, None
+    with open(f) as fh:
+        d = json.load(fh)
+    return d["hbm_read_bytes_per_step"] + d["hbm_write_bytes_per_step"], d["source"]
+
+
 def cpu_baseline(cfg, csr, W_in, u, budget_s=12.0):
     import numpy as np
     from oracle import pyoracle as po
@@ -148,6 +159,7 @@ def main():
     nnz = len(csr[2])
     bps = algorithmic_bytes_per_neuron_step(n, nnz, L)
     achieved = bps * n / (ms_per_step * 1e-3) / 1e9
+    traffic_step, traffic_src = measured_traffic_per_step()
     line = {
         "metric": METRIC,
         "value": value,
@@ -166,8 +178,11 @@ def main():
                    "neurons": n, "memory_len": L, "nnz": nnz, "time_steps": K, "mode": "exact",
                    "record": "state V [T x N] f64"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "algorithmic_bytes_per_step": bps * n},
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": traffic_step * K if traffic_step else None,
+                     "traffic_source": traffic_src,
+                     "algorithmic_bytes_per_launch": bps * n * K,
+                     "kernel": "k_persist_tp<32> (one launch of K steps)"},
         "wall_s": wall,
         "final_state_checksum": float(np.sum(X)),
     }
