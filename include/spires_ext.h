@@ -94,6 +94,31 @@ int spires_ext_agile_optimize(const double *theta0, const double *lo, const doub
                               double (*loss)(const double *theta, int dim, void *ctx), void *ctx,
                               const double *opts, double *theta_best);
 
+/* AGILE surrogate: FP32 MLP in_dim -> hidden -> hidden -> 1 (tanh hidden
+ * layers, linear output) with Adam, forward and backward as HIP kernels on
+ * the device.  No counterpart in the reference: it predicts a trial's
+ * validation score from (hyperparameters, fidelity) so that the search can
+ * prune candidates between fidelity rungs.  Parameters, in order:
+ * W1[hidden x in_dim], b1[hidden], W2[hidden x hidden], b2[hidden],
+ * W3[hidden], b3.  X is [B x in_dim] row-major, targets T and outputs y [B].
+ * The loss is mean((y - T)^2).  in_dim <= 16, hidden <= 64. */
+typedef struct spires_mlp spires_mlp;
+spires_status spires_ext_mlp_create(int in_dim, int hidden, unsigned long long seed, spires_mlp **out);
+void          spires_ext_mlp_destroy(spires_mlp *n);
+int           spires_ext_mlp_num_params(const spires_mlp *n);
+spires_status spires_ext_mlp_get_params(spires_mlp *n, float *params);
+/* Sets the parameters and clears the Adam state. */
+spires_status spires_ext_mlp_set_params(spires_mlp *n, const float *params);
+spires_status spires_ext_mlp_predict(spires_mlp *n, const float *X, int B, float *y);
+/* Gradient of the loss over the batch (grad may be NULL). */
+spires_status spires_ext_mlp_gradient(spires_mlp *n, const float *X, const float *T, int B, float *grad,
+                                      float *loss);
+/* `epochs` full-batch Adam steps (beta1 0.9, beta2 0.999, eps 1e-8); loss of
+ * the last gradient evaluation. */
+spires_status spires_ext_mlp_train(spires_mlp *n, const float *X, const float *T, int B, int epochs, float lr,
+                                   float *loss);
+const char   *spires_ext_mlp_last_error(void);
+
 #ifdef __cplusplus
 }
 #endif

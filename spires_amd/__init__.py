@@ -68,6 +68,7 @@ _D = C.POINTER(C.c_double)
 _I64 = C.POINTER(C.c_int64)
 _I32 = C.POINTER(C.c_int32)
 _U8 = C.POINTER(C.c_uint8)
+_F = C.POINTER(C.c_float)
 _P = C.c_void_p
 _SZ = C.c_size_t
 
@@ -113,6 +114,15 @@ SIGNATURES = {
     "spires_ext_last_error": (C.c_char_p, []),
     "spires_ext_agile_optimize": (C.c_int, [_D, _D, _D, C.c_int, LOSS_FN, _P, _D, _D]),
     "spires_ext_spectral_radius": (C.c_double, [_SZ, _I64, _I32, _D]),
+    "spires_ext_mlp_create": (C.c_int, [C.c_int, C.c_int, C.c_ulonglong, C.POINTER(_P)]),
+    "spires_ext_mlp_destroy": (None, [_P]),
+    "spires_ext_mlp_num_params": (C.c_int, [_P]),
+    "spires_ext_mlp_get_params": (C.c_int, [_P, _F]),
+    "spires_ext_mlp_set_params": (C.c_int, [_P, _F]),
+    "spires_ext_mlp_predict": (C.c_int, [_P, _F, C.c_int, _F]),
+    "spires_ext_mlp_gradient": (C.c_int, [_P, _F, _F, C.c_int, _F, _F]),
+    "spires_ext_mlp_train": (C.c_int, [_P, _F, _F, C.c_int, C.c_int, C.c_float, _F]),
+    "spires_ext_mlp_last_error": (C.c_char_p, []),
     "spires_ext_build_weights": (C.c_int, [C.POINTER(Config), C.POINTER(_SZ), C.POINTER(_I64), C.POINTER(_I32),
                                            C.POINTER(_D), C.POINTER(_D), C.POINTER(_D)]),
 }
@@ -213,6 +223,73 @@ def build_weights(cfg_tuple):
     for p in (rp, ci, val, wi, wo):
         _libc.free(C.cast(p, _P))
     return out
+
+
+class Surrogate:
+    """The AGILE surrogate MLP in_dim -> hidden -> hidden -> 1 on the GPU
+    (spires_ext_mlp_*): forward, backward and Adam run as HIP kernels."""
+
+    def __init__(self, in_dim=7, hidden=64, seed=1):
+        import numpy as np
+        self._np = np
+        self.h = C.c_void_p()
+        rc = lib().spires_ext_mlp_create(in_dim, hidden, seed, C.byref(self.h))
+        if rc:
+            raise RuntimeError("spires_ext_mlp_create: " + lib().spires_ext_mlp_last_error().decode())
+        self.in_dim = in_dim
+        self.count = lib().spires_ext_mlp_num_params(self.h)
+
+    def _f32(self, a):
+        return self._np.ascontiguousarray(a, dtype=self._np.float32)
+
+    def _ck(self, rc, what):
+        if rc:
+            raise RuntimeError(what + ": " + lib().spires_ext_mlp_last_error().decode())
+
+    def params(self):
+        p = self._np.empty(self.count, dtype=self._np.float32)
+        self._ck(lib().spires_ext_mlp_get_params(self.h, p.ctypes.data_as(_F)), "mlp_get_params")
+        return p
+
+    def set_params(self, p):
+        p = self._f32(p)
+        assert p.size == self.count
+        self._ck(lib().spires_ext_mlp_set_params(self.h, p.ctypes.data_as(_F)), "mlp_set_params")
+
+    def predict(self, X):
+        X = self._f32(X).reshape(-1, self.in_dim)
+        y = self._np.empty(X.shape[0], dtype=self._np.float32)
+        self._ck(lib().spires_ext_mlp_predict(self.h, X.ctypes.data_as(_F), X.shape[0], y.ctypes.data_as(_F)),
+                 "mlp_predict")
+        return y
+
+    def gradient(self, X, T):
+        X = self._f32(X).reshape(-1, self.in_dim)
+        T = self._f32(T).reshape(-1)
+        g = self._np.empty(self.count, dtype=self._np.float32)
+        loss = C.c_float()
+        self._ck(lib().spires_ext_mlp_gradient(self.h, X.ctypes.data_as(_F), T.ctypes.data_as(_F), X.shape[0],
+                                               g.ctypes.data_as(_F), C.byref(loss)), "mlp_gradient")
+        return g, loss.value
+
+    def train(self, X, T, epochs=200, lr=1e-3):
+        X = self._f32(X).reshape(-1, self.in_dim)
+        T = self._f32(T).reshape(-1)
+        loss = C.c_float()
+        self._ck(lib().spires_ext_mlp_train(self.h, X.ctypes.data_as(_F), T.ctypes.data_as(_F), X.shape[0],
+                                            epochs, C.c_float(lr), C.byref(loss)), "mlp_train")
+        return loss.value
+
+    def close(self):
+        if self.h:
+            lib().spires_ext_mlp_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class Reservoir:

@@ -55,7 +55,7 @@ void project(double* x, const double* lo, const double* hi, int d, int refl) {
 
 void spsa(const double* th, const double* lo, const double* hi, int d, double eps, loss_fn f, void* ctx,
           uint64_t* st, double* g) {
-    double dl[64], tp[64], tm[64];
+    double dl[64] = {0}, tp[64] = {0}, tm[64] = {0};
     for (int i = 0; i < d; ++i) {
         dl[i] = (pcg32(st) & 1) ? 1.0 : -1.0;
         tp[i] = clampd(th[i] + eps * dl[i], lo[i], hi[i]);
