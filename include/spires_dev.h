@@ -84,6 +84,15 @@ int  spires_dev_upload_inputs(spires_dev *g, const double *u, size_t T);
 #define SPIRES_DEV_REC_OUTPUT 4u   /* write y = W_out V into the output record        */
 /* Record buffers are (re)allocated to hold T rows starting at row 0. */
 int  spires_dev_run(spires_dev *g, size_t t0, size_t T, unsigned flags);
+/* spires_dev_run for `count` exact-mode FLIF-GL reservoirs at once: one
+ * cooperative launch in which each reservoir keeps its own workgroups,
+ * exchange words, inputs and records.  They must all fit the product-wave
+ * kernel with the same neurons per workgroup, and together use no more
+ * workgroups than the device has compute units.  Each reservoir runs macro
+ * steps [t0, t0+T) of its own uploaded input series. */
+int  spires_dev_run_batch(spires_dev *const *gs, int count, size_t t0, size_t T, unsigned flags);
+/* Workgroups a batch would use (0 if the reservoirs cannot run batched). */
+int  spires_dev_batch_workgroups(spires_dev *const *gs, int count);
 /* Pre-size the record buffers (so a later run allocates nothing). */
 int  spires_dev_reserve_records(spires_dev *g, size_t rows, unsigned flags);
 int  spires_dev_read_records(spires_dev *g, size_t row0, size_t rows, double *states,

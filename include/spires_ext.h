@@ -66,6 +66,16 @@ spires_status spires_ext_train_ridge_washout(spires_reservoir *r, const double *
                                              const double *y, size_t T, size_t washout,
                                              double lambda, double *info);
 
+/* Batched runs: `count` reservoirs advance over macro steps [t0, t0+T) of
+ * their own uploaded input series in one device launch, each into its own
+ * records (as spires_ext_run_resident).  They must be exact-mode FLIF-GL
+ * reservoirs of the same history-length class that fit the product-wave
+ * kernel together (spires_ext_batch_workgroups returns 0 otherwise).  No
+ * counterpart in the reference, whose AGILE evaluates trials one by one. */
+spires_status spires_ext_run_resident_batch(spires_reservoir *const *rs, int count, size_t t0, size_t T,
+                                            unsigned flags);
+int           spires_ext_batch_workgroups(spires_reservoir *const *rs, int count);
+
 /* Step arithmetic. */
 #define SPIRES_EXT_MODE_EXACT 0   /* FP64, bit-exact with the reference CPU path (default) */
 #define SPIRES_EXT_MODE_FAST  1   /* FP32 time-blocked GL history on matrix cores          */

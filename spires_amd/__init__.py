@@ -102,6 +102,8 @@ SIGNATURES = {
     "spires_ext_reserve_records": (C.c_int, [_P, _SZ, C.c_uint]),
     "spires_ext_run_resident": (C.c_int, [_P, _SZ, _SZ, C.c_uint]),
     "spires_ext_read_records": (C.c_int, [_P, _SZ, _SZ, _D, _U8, _D]),
+    "spires_ext_run_resident_batch": (C.c_int, [C.POINTER(_P), C.c_int, _SZ, _SZ, C.c_uint]),
+    "spires_ext_batch_workgroups": (C.c_int, [C.POINTER(_P), C.c_int]),
     "spires_ext_train_ridge_washout": (C.c_int, [_P, _D, _D, _SZ, _SZ, C.c_double, _D]),
     "spires_ext_set_mode": (C.c_int, [_P, C.c_int, C.c_int]),
     "spires_ext_set_kernel_path": (C.c_int, [_P, C.c_int]),
@@ -223,6 +225,20 @@ def build_weights(cfg_tuple):
     for p in (rp, ci, val, wi, wo):
         _libc.free(C.cast(p, _P))
     return out
+
+
+def run_resident_batch(reservoirs, t0, T, flags=None):
+    """Advance several reservoirs over steps [t0, t0+T) of their uploaded
+    inputs in one device launch (spires_ext_run_resident_batch)."""
+    flags = REC_STATE if flags is None else flags
+    arr = (_P * len(reservoirs))(*[r.h for r in reservoirs])
+    _check(lib().spires_ext_run_resident_batch(arr, len(reservoirs), t0, T, flags))
+
+
+def batch_workgroups(reservoirs):
+    """Workgroups a batched run of these reservoirs would use (0: cannot batch)."""
+    arr = (_P * len(reservoirs))(*[r.h for r in reservoirs])
+    return lib().spires_ext_batch_workgroups(arr, len(reservoirs))
 
 
 class Surrogate:

@@ -695,6 +695,42 @@ int spires_dev_run(spires_dev* g, size_t t0, size_t T, unsigned flags) {
     return 0;
 }
 
+int spires_dev_run_batch(spires_dev* const* gs, int count, size_t t0, size_t T, unsigned flags) {
+    if (!gs || count < 1) return SPIRES_DEV_EINVAL;
+    std::vector<double*> rv(count);
+    std::vector<uint8_t*> rs(count);
+    unsigned need = flags | ((flags & SPIRES_DEV_REC_OUTPUT) ? SPIRES_DEV_REC_STATE : 0u);
+    for (int r = 0; r < count; ++r) {
+        spires_dev* g = gs[r];
+        if (!g) return SPIRES_DEV_EINVAL;
+        if (g->mode != SPIRES_DEV_MODE_EXACT || g->path == 1) { set_error("batched runs are exact-mode persistent runs"); return SPIRES_DEV_EINVAL; }
+        if (t0 + T > g->u_T && g->din > 0) { set_error("run beyond uploaded inputs"); return SPIRES_DEV_EINVAL; }
+        if (!g->ring64_valid) {
+            int rcs = fast_sync_rings(g, false);
+            if (rcs) return rcs;
+            g->ring64_valid = 1;
+        }
+        g->ring32_valid = 0;
+        int rc = ensure_records(g, T, need);
+        if (rc) return rc;
+        rv[r] = (need & SPIRES_DEV_REC_STATE) ? g->rec_state : nullptr;
+        rs[r] = (flags & SPIRES_DEV_REC_RASTER) ? g->rec_raster : nullptr;
+    }
+    if (T == 0) return 0;
+    int rc = launch_persist_batch(gs, count, (long long)t0, (long long)T, rv.data(), rs.data());
+    if (rc) return rc;
+    if (flags & SPIRES_DEV_REC_OUTPUT)
+        for (int r = 0; r < count; ++r)
+            if ((rc = outputs_from_states(gs[r], gs[r]->rec_state, T, gs[r]->rec_out))) return rc;
+    return 0;
+}
+
+int spires_dev_batch_workgroups(spires_dev* const* gs, int count) {
+    int total = 0;
+    if (!gs || count < 1 || batch_plan(gs, count, nullptr, &total)) return 0;
+    return total;
+}
+
 int spires_dev_reserve_records(spires_dev* g, size_t rows, unsigned flags) {
     if (!g) return SPIRES_DEV_EINVAL;
     return ensure_records(g, rows, flags);

@@ -117,6 +117,10 @@ void readout_release(spires_dev* g);
 int persist_supported(spires_dev* g);
 int launch_persist(spires_dev* g, const double* u_series, long long t0, long long T, double* rec_V,
                    uint8_t* rec_S);
+// Several exact-mode FLIF-GL reservoirs in one launch of the product-wave kernel.
+int batch_plan(spires_dev* const* gs, int count, int* nb_out, int* wgs_out);
+int launch_persist_batch(spires_dev* const* gs, int count, long long t0, long long T, double* const* rec_V,
+                         uint8_t* const* rec_S);
 int outputs_from_states(spires_dev* g, const double* states, size_t T, double* out);
 int fast_supported(spires_dev* g);
 int fast_sync_rings(spires_dev* g, bool to_fast);

@@ -412,13 +412,13 @@ __device__ __forceinline__ void lds_counter_store(unsigned* f, unsigned v) {
 __host__ __device__ __forceinline__ int tp_row_stride(int L) { return 32 * ((L - 2 + 31) / 32) + 2; }
 
 template <int NB>
-__global__ void __launch_bounds__(kTpThreads) k_persist_tp(PersistArgs a) {
+__device__ __forceinline__ void tp_body(const PersistArgs& a, const int g) {
     constexpr int C = kTpChunk;
     constexpr int PARTS = 64 * kTpProducers / NB;   // producer lanes per neuron
     constexpr int PER = C / PARTS;                  // lags per producer lane per chunk
     static_assert(PER >= 2 && PER % 2 == 0, "producer lanes write lag pairs");
     extern __shared__ __align__(16) double lds[];   // 16-byte aligned: b128 product reads
-    const int g = blockIdx.x, tid = threadIdx.x, lane = tid & 63;
+    const int tid = threadIdx.x, lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int i0 = g * NB;
     const int nloc = min(NB, a.n - i0);
@@ -682,6 +682,21 @@ __global__ void __launch_bounds__(kTpThreads) k_persist_tp(PersistArgs a) {
     }
 }
 
+template <int NB>
+__global__ void __launch_bounds__(kTpThreads) k_persist_tp(PersistArgs a) {
+    tp_body<NB>(a, blockIdx.x);
+}
+
+// Several reservoirs in one launch: workgroup b runs local workgroup
+// wg_map[b].y of reservoir wg_map[b].x with that reservoir's arguments.  The
+// reservoirs share nothing but the launch (each has its own exchange words).
+template <int NB>
+__global__ void __launch_bounds__(kTpThreads) k_persist_tp_batch(const PersistArgs* __restrict__ table,
+                                                                 const int2* __restrict__ wg_map) {
+    const int2 mw = wg_map[blockIdx.x];
+    tp_body<NB>(table[mw.x], mw.y);
+}
+
 namespace {
 
 struct Plan {
@@ -776,10 +791,11 @@ int persist_supported(spires_dev* g) {
     return persist_plan(g, num_cus(), &pl) ? 1 : 0;
 }
 
-int launch_persist(spires_dev* g, const double* u_series, long long t0, long long T, double* rec_V,
-                   uint8_t* rec_S) {
-    Plan pl;
-    if (!persist_plan(g, num_cus(), &pl)) { set_error("persistent path unsupported"); return SPIRES_DEV_EINVAL; }
+namespace {
+
+// Exchange words (cleared when the workgroup count changes) and the abort flag
+// of one reservoir's launch.
+int prepare_launch(spires_dev* g, const Plan& pl) {
     const int wpg = (pl.nb + 31) / 32;
     const size_t nw = (size_t)pl.G * wpg;
     if (g->xch_words < 2 * nw) {
@@ -798,9 +814,14 @@ int launch_persist(spires_dev* g, const double* u_series, long long t0, long lon
         if (rc) return rc;
     }
     SDEV_CK(hipMemsetAsync(g->abort_flag, 0, 4, g->stream));
+    return 0;
+}
+
+PersistArgs make_args(spires_dev* g, const Plan& pl, const double* u_series, long long t0, long long T,
+                      double* rec_V, uint8_t* rec_S) {
     PersistArgs a;
     memset(&a, 0, sizeof(a));
-    a.n = g->n; a.din = g->din; a.L = g->L; a.G = pl.G; a.wpg = wpg; a.micro = g->micro;
+    a.n = g->n; a.din = g->din; a.L = g->L; a.G = pl.G; a.wpg = (pl.nb + 31) / 32; a.micro = g->micro;
     a.stype = g->stype; a.ne_lds = pl.ne_lds;
     a.st0 = g->step; a.t0 = t0; a.T = T;
     a.gain = g->d.gain; a.dt = g->d.dt;
@@ -812,6 +833,30 @@ int launch_persist(spires_dev* g, const double* u_series, long long t0, long lon
     a.coef = g->coef;
     a.xch = g->xch; a.abort_flag = g->abort_flag;
     a.rec_V = rec_V; a.rec_S = rec_S;
+    return a;
+}
+
+// After a launch: advance the step counters and report an aborted exchange.
+int finish_launch(spires_dev* g, long long T) {
+    g->step += (long long)T * g->micro;
+    g->msteps += (size_t)T;
+    g->cur ^= 1;
+    unsigned flag = 0;
+    SDEV_CK(hipMemcpyAsync(&flag, g->abort_flag, 4, hipMemcpyDeviceToHost, g->stream));
+    SDEV_CK(hipStreamSynchronize(g->stream));
+    if (flag) { set_error("persistent step kernel timed out waiting for the spike exchange"); return SPIRES_DEV_EINTERNAL; }
+    return 0;
+}
+
+}  // namespace
+
+int launch_persist(spires_dev* g, const double* u_series, long long t0, long long T, double* rec_V,
+                   uint8_t* rec_S) {
+    Plan pl;
+    if (!persist_plan(g, num_cus(), &pl)) { set_error("persistent path unsupported"); return SPIRES_DEV_EINVAL; }
+    int rc = prepare_launch(g, pl);
+    if (rc) return rc;
+    PersistArgs a = make_args(g, pl, u_series, t0, T, rec_V, rec_S);
     void* args[] = {&a};
     const void* fn = nullptr;
     int threads = kPersistThreads;
@@ -827,13 +872,72 @@ int launch_persist(spires_dev* g, const double* u_series, long long t0, long lon
     }
     SDEV_CK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)pl.lds));
     SDEV_CK(hipLaunchCooperativeKernel(fn, dim3(pl.G), dim3(threads), args, (unsigned)pl.lds, g->stream));
-    g->step += (long long)T * g->micro;
-    g->msteps += (size_t)T;
-    g->cur ^= 1;
-    unsigned flag = 0;
-    SDEV_CK(hipMemcpyAsync(&flag, g->abort_flag, 4, hipMemcpyDeviceToHost, g->stream));
-    SDEV_CK(hipStreamSynchronize(g->stream));
-    if (flag) { set_error("persistent step kernel timed out waiting for the spike exchange"); return SPIRES_DEV_EINTERNAL; }
+    return finish_launch(g, T);
+}
+
+int batch_plan(spires_dev* const* gs, int count, int* nb_out, int* wgs_out) {
+    const int ncu = num_cus();
+    int nb = 0, total = 0;
+    for (int r = 0; r < count; ++r) {
+        Plan pl;
+        if (!persist_plan(gs[r], ncu, &pl) || !pl.tp) {
+            set_error("batched runs need FLIF-GL reservoirs that fit the product-wave kernel");
+            return SPIRES_DEV_EINVAL;
+        }
+        if (r == 0) nb = pl.nb;
+        if (pl.nb != nb) { set_error("batched reservoirs need the same neurons-per-workgroup (history length class)"); return SPIRES_DEV_EINVAL; }
+        total += pl.G;
+    }
+    if (total > ncu) { set_error("batched reservoirs need more workgroups than the device has compute units"); return SPIRES_DEV_EINVAL; }
+    if (nb_out) *nb_out = nb;
+    if (wgs_out) *wgs_out = total;
+    return 0;
+}
+
+int launch_persist_batch(spires_dev* const* gs, int count, long long t0, long long T, double* const* rec_V,
+                         uint8_t* const* rec_S) {
+    int nb = 0, total = 0;
+    int rc = batch_plan(gs, count, &nb, &total);
+    if (rc) return rc;
+    const int ncu = num_cus();
+    std::vector<PersistArgs> table(count);
+    std::vector<int2> map;
+    map.reserve(total);
+    size_t lds = 0;
+    for (int r = 0; r < count; ++r) {
+        spires_dev* g = gs[r];
+        Plan pl;
+        persist_plan(g, ncu, &pl);
+        if ((rc = prepare_launch(g, pl))) return rc;
+        SDEV_CK(hipStreamSynchronize(g->stream));
+        table[r] = make_args(g, pl, g->din > 0 ? g->u_series : nullptr, t0, T, rec_V[r], rec_S[r]);
+        for (int b = 0; b < pl.G; ++b) map.push_back(make_int2(r, b));
+        lds = std::max(lds, pl.lds);
+    }
+    spires_dev* g0 = gs[0];
+    PersistArgs* table_d = nullptr;
+    int2* map_d = nullptr;
+    SDEV_CK(hipMalloc(&table_d, table.size() * sizeof(PersistArgs)));
+    if (hipMalloc(&map_d, map.size() * sizeof(int2)) != hipSuccess) {
+        (void)hipFree(table_d);
+        set_error("device allocation failed");
+        return SPIRES_DEV_EALLOC;
+    }
+    hipError_t e = hipMemcpyAsync(table_d, table.data(), table.size() * sizeof(PersistArgs), hipMemcpyHostToDevice,
+                                  g0->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(map_d, map.data(), map.size() * sizeof(int2), hipMemcpyHostToDevice, g0->stream);
+    const void* fn = nb == 32 ? (const void*)k_persist_tp_batch<32> : (const void*)k_persist_tp_batch<16>;
+    if (e == hipSuccess) e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    void* args[] = {&table_d, &map_d};
+    if (e == hipSuccess)
+        e = hipLaunchCooperativeKernel(fn, dim3(total), dim3(kTpThreads), args, (unsigned)lds, g0->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(g0->stream);
+    (void)hipFree(table_d);
+    (void)hipFree(map_d);
+    if (e != hipSuccess) { set_error(hipGetErrorString(e)); return SPIRES_DEV_EINTERNAL; }
+    for (int r = 0; r < count; ++r)
+        if ((rc = finish_launch(gs[r], T))) return rc;
     return 0;
 }
 

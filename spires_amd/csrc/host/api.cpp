@@ -621,6 +621,27 @@ spires_status spires_ext_run_resident(spires_reservoir* r, size_t t0, size_t T, 
     return dev_status(spires_dev_run(r->dev, t0, T, flags));
 }
 
+spires_status spires_ext_run_resident_batch(spires_reservoir* const* rs, int count, size_t t0, size_t T,
+                                            unsigned flags) {
+    if (!rs || count < 1) return SPIRES_ERR_INVALID_ARG;
+    std::vector<spires_dev*> devs(count);
+    for (int i = 0; i < count; ++i) {
+        if (!rs[i] || !rs[i]->dev) return SPIRES_ERR_INVALID_ARG;
+        devs[i] = rs[i]->dev;
+    }
+    return dev_status(spires_dev_run_batch(devs.data(), count, t0, T, flags));
+}
+
+int spires_ext_batch_workgroups(spires_reservoir* const* rs, int count) {
+    if (!rs || count < 1) return 0;
+    std::vector<spires_dev*> devs(count);
+    for (int i = 0; i < count; ++i) {
+        if (!rs[i] || !rs[i]->dev) return 0;
+        devs[i] = rs[i]->dev;
+    }
+    return spires_dev_batch_workgroups(devs.data(), count);
+}
+
 spires_status spires_ext_read_records(spires_reservoir* r, size_t row0, size_t rows, double* states,
                                       uint8_t* raster, double* outputs) {
     if (!r || !r->dev) return SPIRES_ERR_INVALID_ARG;
