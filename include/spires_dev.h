@@ -93,6 +93,11 @@ int  spires_dev_run(spires_dev *g, size_t t0, size_t T, unsigned flags);
 int  spires_dev_run_batch(spires_dev *const *gs, int count, size_t t0, size_t T, unsigned flags);
 /* Workgroups a batch would use (0 if the reservoirs cannot run batched). */
 int  spires_dev_batch_workgroups(spires_dev *const *gs, int count);
+/* Exact-mode launch plan: info[0] kernel (0 per-step launches, 1 single-wave
+ * persistent, 2 product-wave persistent), [1] neurons per workgroup,
+ * [2] workgroups, [3] row pull (0 edge list, 1 spike-word bitmaps),
+ * [4] recurrent weights held in LDS, [5] LDS bytes per workgroup. */
+int  spires_dev_plan_info(spires_dev *g, int *info);
 /* Pre-size the record buffers (so a later run allocates nothing). */
 int  spires_dev_reserve_records(spires_dev *g, size_t rows, unsigned flags);
 int  spires_dev_read_records(spires_dev *g, size_t row0, size_t rows, double *states,
@@ -112,6 +117,9 @@ int  spires_dev_set_readout(spires_dev *g, const double *W_out);
  * residual of the normal equations. */
 int  spires_dev_ridge(spires_dev *g, size_t washout, const double *Y, double lambda,
                       double *info_out);
+/* As spires_dev_ridge on the recorded rows [row0, row1) with Y[(row1-row0) x dout]. */
+int  spires_dev_ridge_rows(spires_dev *g, size_t row0, size_t row1, const double *Y, double lambda,
+                           double *info_out);
 /* Recursive least squares over T resident input steps with targets Y[T x dout]. */
 int  spires_dev_rls(spires_dev *g, const double *Y, size_t T, double delta, double lambda);
 /* Delta rule on the current state. */

@@ -66,15 +66,25 @@ spires_status spires_ext_train_ridge_washout(spires_reservoir *r, const double *
                                              const double *y, size_t T, size_t washout,
                                              double lambda, double *info);
 
+/* Ridge readout fitted on recorded states rows [row0, row1) of the last
+ * resident run (no re-run), against Y[(row1-row0) x Dout]; info as in
+ * spires_ext_train_ridge_washout. */
+spires_status spires_ext_fit_ridge_records(spires_reservoir *r, size_t row0, size_t row1, const double *Y,
+                                           double lambda, double *info);
+
 /* Batched runs: `count` reservoirs advance over macro steps [t0, t0+T) of
  * their own uploaded input series in one device launch, each into its own
  * records (as spires_ext_run_resident).  They must be exact-mode FLIF-GL
- * reservoirs of the same history-length class that fit the product-wave
- * kernel together (spires_ext_batch_workgroups returns 0 otherwise).  No
+ * reservoirs with the same neurons per workgroup (spires_ext_kernel_plan)
+ * that fit the product-wave kernel together (spires_ext_batch_workgroups returns 0 otherwise).  No
  * counterpart in the reference, whose AGILE evaluates trials one by one. */
 spires_status spires_ext_run_resident_batch(spires_reservoir *const *rs, int count, size_t t0, size_t T,
                                             unsigned flags);
 int           spires_ext_batch_workgroups(spires_reservoir *const *rs, int count);
+/* The exact-mode launch plan of a reservoir (spires_dev_plan_info layout:
+ * kernel, neurons per workgroup, workgroups, row-pull kind, weights in LDS,
+ * LDS bytes).  No counterpart in the reference. */
+spires_status spires_ext_kernel_plan(spires_reservoir *r, int *info);
 
 /* Step arithmetic. */
 #define SPIRES_EXT_MODE_EXACT 0   /* FP64, bit-exact with the reference CPU path (default) */

@@ -103,7 +103,9 @@ SIGNATURES = {
     "spires_ext_run_resident": (C.c_int, [_P, _SZ, _SZ, C.c_uint]),
     "spires_ext_read_records": (C.c_int, [_P, _SZ, _SZ, _D, _U8, _D]),
     "spires_ext_run_resident_batch": (C.c_int, [C.POINTER(_P), C.c_int, _SZ, _SZ, C.c_uint]),
+    "spires_ext_fit_ridge_records": (C.c_int, [_P, _SZ, _SZ, _D, C.c_double, _D]),
     "spires_ext_batch_workgroups": (C.c_int, [C.POINTER(_P), C.c_int]),
+    "spires_ext_kernel_plan": (C.c_int, [_P, C.POINTER(C.c_int)]),
     "spires_ext_train_ridge_washout": (C.c_int, [_P, _D, _D, _SZ, _SZ, C.c_double, _D]),
     "spires_ext_set_mode": (C.c_int, [_P, C.c_int, C.c_int]),
     "spires_ext_set_kernel_path": (C.c_int, [_P, C.c_int]),
@@ -382,6 +384,14 @@ class Reservoir:
     def run_resident(self, t0, T, flags=0):
         _check(lib().spires_ext_run_resident(self.h, t0, T, flags))
 
+    def fit_ridge_records(self, row0, row1, Y, lam):
+        """Ridge W_out on recorded state rows [row0, row1) against Y; returns
+        (used_lu, relative residual)."""
+        Y = _f64(Y).reshape(row1 - row0, -1)
+        info = np.zeros(2)
+        _check(lib().spires_ext_fit_ridge_records(self.h, row0, row1, _dp(Y), lam, _dp(info)))
+        return bool(info[0]), float(info[1])
+
     def read_records(self, row0, rows, states=False, raster=False, outputs=False):
         X = np.empty((rows, self.N)) if states else None
         R = np.empty((rows, self.N), np.uint8) if raster else None
@@ -398,6 +408,12 @@ class Reservoir:
 
     def set_mode(self, mode, block_k=0):
         _check(lib().spires_ext_set_mode(self.h, mode, block_k))
+
+    def kernel_plan(self):
+        """Exact-mode launch plan (spires_ext_kernel_plan) as a dict."""
+        info = (C.c_int * 6)()
+        _check(lib().spires_ext_kernel_plan(self.h, info))
+        return dict(zip(("kernel", "nb", "workgroups", "row_pull", "weights_lds", "lds_bytes"), list(info)))
 
     def set_kernel_path(self, path):
         """0 automatic, 1 one launch per micro-step, 2 persistent kernel only."""

@@ -506,7 +506,7 @@ void spires_dev_destroy(spires_dev* g) {
     dfree(g->edge_decay); dfree(g->edge_tr); dfree(g->frac_coef); dfree(g->frac_hist);
     dfree(g->frac_I); dfree(g->me_tr); dfree(g->me_comb); dfree(g->u_series);
     dfree(g->rec_state); dfree(g->rec_raster); dfree(g->rec_out); dfree(g->ybuf); dfree(g->ustep);
-    dfree(g->xch); dfree(g->abort_flag); dfree(g->ring32);
+    dfree(g->xch); dfree(g->abort_flag); dfree(g->ring32); dfree(g->rbm);
     if (g->stream) (void)hipStreamDestroy(g->stream);
     delete g;
 }
@@ -556,6 +556,7 @@ int spires_dev_create(const spires_dev_desc* d, spires_dev** out) {
     if (g->backend == BK_SPARSE) {
         g->nnz = d->nnz;
         g->rp_host.assign(d->row_ptr, d->row_ptr + n + 1);
+        g->col_host.assign(d->col, d->col + d->nnz);
         if ((rc = dalloc(&g->rp, n + 1)) || (rc = dalloc(&g->col, (size_t)d->nnz)) || (rc = dalloc(&g->val, (size_t)d->nnz)))
             return fail(rc);
         if ((rc = up(g->rp, d->row_ptr, (n + 1) * 8)) || (rc = up(g->col, d->col, (size_t)d->nnz * 4)) ||

@@ -134,4 +134,75 @@ __device__ __forceinline__ double row_pull(int stype, bool csr_lds, int rs, int 
     return s;
 }
 
+// Row pull over column bitmaps (SIMPLE synapses, dense rows).  Row r of a
+// workgroup keeps bm[r][w], the set of its columns among neurons 32w..32w+31,
+// and the spikes come as words of the same layout; a row's CSR edges are its
+// bitmap's set bits in ascending (word, bit) order, which is ascending column
+// order.  The pull runs in two passes of one wave:
+//   scan  all 64 lanes, lane = part * nb + r: the part's words [part*cw,
+//         part*cw + cw) of row r are ANDed with the spike words and the
+//         block-relative indices of the active edges are listed in place
+//         (slot <= edge index, so parts never overlap);
+//   sum   lane r adds the listed weights of its parts in order, the loads of
+//         each batch issued before its adds.
+// The sum therefore visits the spiking columns in ascending order, as the
+// edge-list pull does.  pb[lane] is the block-relative index of the first
+// edge of the lane's part (fixed for a launch).
+__device__ __forceinline__ void bitmap_scan(const unsigned* __restrict__ bm_s, const unsigned* __restrict__ bits_s,
+                                            unsigned short* __restrict__ list_s, int* __restrict__ cnt_s,
+                                            const int* __restrict__ pb_s, int nw, int cw, int nb, bool live,
+                                            int lane) {
+    const int r = lane & (nb - 1), part = lane / nb;
+    const int w_lo = part * cw, w_hi = min(nw, w_lo + cw);
+    const unsigned* row = bm_s + r * nw;
+    const int first = pb_s[lane];
+    int pos = first, c = first;
+    if (live) {
+        for (int w0 = w_lo; w0 < w_hi; w0 += 8) {
+            unsigned m[8], sw[8];
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                const bool ok = w0 + q < w_hi;
+                m[q] = ok ? row[w0 + q] : 0u;
+                sw[q] = ok ? bits_s[w0 + q] : 0u;
+            }
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                unsigned act = m[q] & sw[q];
+                while (act) {
+                    const unsigned b = (unsigned)__builtin_ctz(act);
+                    list_s[c++] = (unsigned short)(pos + __builtin_popcount(m[q] & ((1u << b) - 1u)));
+                    act &= act - 1u;
+                }
+                pos += __builtin_popcount(m[q]);
+            }
+        }
+    }
+    cnt_s[lane] = c - first;
+}
+
+// B weights loaded per batch; the list holds >= B valid indices past any
+// part's count (stale entries or the initial zeros), so every load is in range.
+template <int B>
+__device__ __forceinline__ double bitmap_sum(const unsigned short* __restrict__ list_s, const int* __restrict__ cnt_s,
+                                             const int* __restrict__ pb_s, const double* __restrict__ w, int nb,
+                                             int lane) {
+    double s = 0.0;
+    for (int part = 0; part < 64 / nb; ++part) {
+        const int c = cnt_s[part * nb + lane], b0 = pb_s[part * nb + lane];
+        for (int j = 0; j < c; j += B) {
+            unsigned idx[B];
+            double wv[B];
+#pragma unroll
+            for (int q = 0; q < B; ++q) idx[q] = list_s[b0 + j + q];
+#pragma unroll
+            for (int q = 0; q < B; ++q) wv[q] = w[idx[q]];
+#pragma unroll
+            for (int q = 0; q < B; ++q)
+                if (j + q < c) s = s + wv[q];
+        }
+    }
+    return s;
+}
+
 }  // namespace sdev

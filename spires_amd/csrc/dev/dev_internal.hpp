@@ -80,6 +80,8 @@ struct spires_dev {
     double* W_in = nullptr; double* W_out = nullptr;
     int64_t* rp = nullptr; int32_t* col = nullptr; double* val = nullptr; int64_t nnz = 0;
     std::vector<int64_t> rp_host;   // host copy of the row pointers (launch planning)
+    std::vector<int32_t> col_host;  // host copy of the column indices (row bitmaps)
+    unsigned* rbm = nullptr;        // per-row column bitmaps [n][ceil(n/32)] (bitmap row pull)
     double* WT = nullptr;       // dense backend, transposed (column j contiguous over rows)
     // neuron state
     double* coef = nullptr; int ncoef = 0;

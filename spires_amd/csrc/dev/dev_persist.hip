@@ -41,6 +41,7 @@ constexpr int kPersistThreads = 128;
 struct PersistArgs {
     int n, din, L, G, wpg, micro, stype;
     int ne_lds;                   // LDS capacity for CSR edges per workgroup (0: read CSR from HBM)
+    int rpm, cw;                  // row pull: 0 edge list, 1 spike-word bitmaps (cw words per scan part)
     long long st0;                // micro-step index since reset at launch start
     long long t0, T;              // macro steps [t0, t0+T) of the resident input series
     double gain, dt;
@@ -53,6 +54,7 @@ struct PersistArgs {
     unsigned long long* xch;      // [2][G*wpg]
     unsigned* abort_flag;
     double* rec_V; uint8_t* rec_S;
+    const unsigned* rbm;          // [n][ceil(n/32)] row bitmaps in column order (rpm 1)
 };
 
 // FP64 multiply and add with their issue order pinned: each is one IEEE
@@ -399,6 +401,7 @@ constexpr int kTpThreads = 384;
 constexpr int kTpChunk = 32;
 constexpr int kTpProducers = 4;
 constexpr long long kTpSpinLimit = 1ll << 24;
+constexpr int kBitmapBatch = 16;     // weights loaded per batch of the bitmap pull from HBM (half from LDS)
 
 // Chain progress: 64 * (micro-step in launch) + chunks done; compared modulo 2^32.
 __device__ __forceinline__ unsigned lds_counter_load(const unsigned* f) {
@@ -432,7 +435,13 @@ __device__ __forceinline__ void tp_body(const PersistArgs& a, const int g) {
     double* ext_s = rec_s + 2 * NB;                    // [2][NB]
     double* eval_s = ext_s + 2 * NB;                   // [ne_lds]
     unsigned* bits_s = (unsigned*)(eval_s + a.ne_lds); // [nw]
-    unsigned* eidx_s = bits_s + nw;                    // [ne_lds]
+    unsigned* eidx_s = bits_s + nw;                    // edge list: [ne_lds]
+    const int nwp = (n + 31) >> 5;                     // spike words in column order
+    unsigned* bm_s = bits_s + nw;                      // bitmaps: [NB][nwp]
+    unsigned* pk_s = NB == 32 ? bits_s : bm_s + NB * nwp;   //     [nwp] spikes in column order
+    int* cnt_s = (int*)(bm_s + NB * nwp + (NB == 32 ? 0 : nwp));   // [64]
+    int* pb_s = cnt_s + 64;                            //          [64]
+    unsigned short* list_s = (unsigned short*)(pb_s + 64);   //    [ne + kBitmapBatch]
     __shared__ int abort_s;
     __shared__ unsigned cons_s;
     if (tid == 0) { abort_s = 0; cons_s = 0u; }
@@ -447,11 +456,30 @@ __device__ __forceinline__ void tp_body(const PersistArgs& a, const int g) {
     const int64_t E0 = a.rp[i0];
     const int ne = (int)(a.rp[i0 + nloc] - E0);
     const bool csr_lds = a.ne_lds > 0;
-    if (csr_lds)
+    const bool bitmap = a.rpm == 1;
+    if (bitmap) {
+        for (int e = tid; e < NB * nwp; e += kTpThreads) {
+            const int rr = e / nwp;
+            bm_s[e] = rr < nloc ? a.rbm[(size_t)(i0 + rr) * nwp + (e - rr * nwp)] : 0u;
+        }
+        for (int e = tid; e < ne + kBitmapBatch; e += kTpThreads) list_s[e] = 0;
+        if (csr_lds)
+            for (int e = tid; e < ne; e += kTpThreads) eval_s[e] = a.val[E0 + e];
+        if (wave == 4) {   // first edge of each (part, row) scan range
+            const int rr = lane & (NB - 1), w_end = min(nwp, (lane / NB) * a.cw);
+            int pb = 0;
+            if (rr < nloc) {
+                pb = (int)(a.rp[i0 + rr] - E0);
+                for (int w = 0; w < w_end; ++w) pb += __builtin_popcount(a.rbm[(size_t)(i0 + rr) * nwp + w]);
+            }
+            pb_s[lane] = pb;
+        }
+    } else if (csr_lds) {
         for (int e = tid; e < ne; e += kTpThreads) {
             eidx_s[e] = encode_col(a.col[E0 + e], NB, wpg);
             eval_s[e] = a.val[E0 + e];
         }
+    }
     const int r = lane % NB;                           // neuron row of this lane (chain, producers)
     const int ir = i0 + (r < nloc ? r : nloc - 1);     // clamped global index for reads
     const bool own = (wave == 0) && (lane < nloc);
@@ -585,9 +613,28 @@ __device__ __forceinline__ void tp_body(const PersistArgs& a, const int g) {
                 __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
                 __builtin_amdgcn_wave_barrier();
                 __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                if (bitmap && !aborted) {
+                    if (NB == 16) {   // two 16-neuron exchange words per column-order word
+                        for (int w = lane; w < nwp; w += 64)
+                            pk_s[w] = bits_s[2 * w] | (2 * w + 1 < nw ? bits_s[2 * w + 1] << 16 : 0u);
+                        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                        __builtin_amdgcn_wave_barrier();
+                        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                    }
+                    bitmap_scan(bm_s, pk_s, list_s, cnt_s, pb_s, nwp, a.cw, NB, (lane & (NB - 1)) < nloc, lane);
+                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                    __builtin_amdgcn_wave_barrier();
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                }
                 if (!aborted && lane < nloc) {
-                    rec_s[sb * NB + lane] = row_pull(a.stype, csr_lds, rs, re, E0, eidx_s, eval_s, a.col, a.val,
-                                                     bits_s, NB, wpg, a.edge_decay, a.edge_tr);
+                    double rec;
+                    if (bitmap)
+                        rec = csr_lds ? bitmap_sum<kBitmapBatch / 2>(list_s, cnt_s, pb_s, eval_s, NB, lane)
+                                      : bitmap_sum<kBitmapBatch / 2>(list_s, cnt_s, pb_s, a.val + E0, NB, lane);
+                    else
+                        rec = row_pull(a.stype, csr_lds, rs, re, E0, eidx_s, eval_s, a.col, a.val, bits_s, NB, wpg,
+                                       a.edge_decay, a.edge_tr);
+                    rec_s[sb * NB + lane] = rec;
                     if (mi == 0) {
                         double su = 0.0;
                         if (win_reg) {
@@ -702,6 +749,7 @@ namespace {
 struct Plan {
     int nb = 0, G = 0, ring_lds = 0, ne_lds = 0;
     int tp = 0;                 // FLIF-GL with the trailing product stage (k_persist_tp)
+    int rpm = 0, cw = 0;        // row pull over spike-word bitmaps (k_persist_tp only)
     size_t lds = 0;
 };
 
@@ -732,7 +780,10 @@ bool persist_plan(spires_dev* g, int ncu, Plan* pl) {
         pl->lds = base + (size_t)pl->ne_lds * 12;
         return true;
     };
-    auto fill_tp = [&](int nb) -> bool {
+    // Product-wave kernel.  Row pull: spike-word bitmaps when a workgroup's rows
+    // hold more edges than half the words a bitmap scan reads, else the edge
+    // list; weights in LDS when they fit ("fast"), else read from HBM.
+    auto fill_tp = [&](int nb, bool fast_only) -> bool {
         const int G = (n + nb - 1) / nb;
         const int wpg = (nb + 31) / 32;
         if (L < 2 || G > ncu || G * wpg > 512) return false;
@@ -740,17 +791,39 @@ bool persist_plan(spires_dev* g, int ncu, Plan* pl) {
         const size_t base = PS * nb * 8 + PS * 8 + 6 * (size_t)nb * 8 + (size_t)G * wpg * 4 + 64;
         if (base > cap) return false;
         const int64_t me = max_edges(nb);
-        pl->ne_lds = (base + (size_t)me * 12 <= cap) ? (int)std::max<int64_t>(me, 1) : 0;
-        pl->nb = nb; pl->G = G; pl->ring_lds = 0; pl->tp = 1;
+        const int nwp = (n + 31) / 32;
+        pl->nb = nb; pl->G = G; pl->ring_lds = 0; pl->tp = 1; pl->rpm = 0; pl->cw = 0;
+        const bool dense = g->stype == ST_SIMPLE && me + kBitmapBatch < 65536 && 2 * me > (int64_t)nb * nwp;
+        if (dense) {
+            const size_t bm = ((size_t)(nb + (nb == 32 ? 0 : 1)) * nwp * 4 + 512 + ((size_t)me + kBitmapBatch) * 2 + 7) /
+                              8 * 8;
+            if (base + bm <= cap) {
+                const bool wl = base + bm + (size_t)me * 8 <= cap;
+                if (fast_only && !wl) return false;
+                const int parts = 64 / nb;
+                pl->rpm = 1;
+                pl->cw = ((nwp + parts - 1) / parts + 7) / 8 * 8;
+                pl->ne_lds = wl ? (int)std::max<int64_t>(me, 1) : 0;
+                pl->lds = base + bm + (size_t)pl->ne_lds * 8;
+                return true;
+            }
+        }
+        const bool el = base + (size_t)me * 12 <= cap;
+        if (fast_only && !el) return false;
+        pl->ne_lds = el ? (int)std::max<int64_t>(me, 1) : 0;
         pl->lds = base + (size_t)pl->ne_lds * 12;
         return true;
     };
     if (gl) {
-        // path 2 / auto: trailing-product kernel, then the single-wave chain with the
+        // path 2 / auto: trailing-product kernel (16 neurons per workgroup when only
+        // that keeps the row pull in LDS), then the single-wave chain with the
         // history in LDS, then in HBM; path 3 / 4 force the single-wave HBM / LDS variants.
-        if (g->path == 0 || g->path == 2)
+        if (g->path == 0 || g->path == 2) {
             for (int nb : {32, 16})
-                if (fill_tp(nb)) return true;
+                if (fill_tp(nb, true)) return true;
+            for (int nb : {32, 16})
+                if (fill_tp(nb, false)) return true;
+        }
         if (g->path != 3)
             for (int nb : {32, 16})
                 if (fill(nb, 1)) return true;
@@ -813,6 +886,20 @@ int prepare_launch(spires_dev* g, const Plan& pl) {
         int rc = dalloc(&g->abort_flag, 1);
         if (rc) return rc;
     }
+    if (pl.rpm == 1 && !g->rbm) {
+        // row i: bit j % 32 of word j / 32 for each column j
+        const size_t nwp = ((size_t)g->n + 31) / 32;
+        std::vector<unsigned> bm((size_t)g->n * nwp, 0u);
+        for (int i = 0; i < g->n; ++i)
+            for (int64_t k = g->rp_host[i]; k < g->rp_host[i + 1]; ++k) {
+                const int j = g->col_host[k];
+                bm[(size_t)i * nwp + (j >> 5)] |= 1u << (j & 31);
+            }
+        int rc = dalloc(&g->rbm, bm.size());
+        if (rc) return rc;
+        SDEV_CK(hipMemcpyAsync(g->rbm, bm.data(), bm.size() * 4, hipMemcpyHostToDevice, g->stream));
+        SDEV_CK(hipStreamSynchronize(g->stream));
+    }
     SDEV_CK(hipMemsetAsync(g->abort_flag, 0, 4, g->stream));
     return 0;
 }
@@ -823,6 +910,7 @@ PersistArgs make_args(spires_dev* g, const Plan& pl, const double* u_series, lon
     memset(&a, 0, sizeof(a));
     a.n = g->n; a.din = g->din; a.L = g->L; a.G = pl.G; a.wpg = (pl.nb + 31) / 32; a.micro = g->micro;
     a.stype = g->stype; a.ne_lds = pl.ne_lds;
+    a.rpm = pl.rpm; a.cw = pl.cw; a.rbm = g->rbm;
     a.st0 = g->step; a.t0 = t0; a.T = T;
     a.gain = g->d.gain; a.dt = g->d.dt;
     memcpy(a.np, g->d.np, sizeof(a.np));
@@ -942,3 +1030,17 @@ int launch_persist_batch(spires_dev* const* gs, int count, long long t0, long lo
 }
 
 }  // namespace sdev
+
+extern "C" int spires_dev_plan_info(spires_dev* g, int* info) {
+    if (!g || !info) return SPIRES_DEV_EINVAL;
+    for (int k = 0; k < 6; ++k) info[k] = 0;
+    sdev::Plan pl;
+    if (g->mode != SPIRES_DEV_MODE_EXACT || g->path == 1 || !sdev::persist_plan(g, sdev::num_cus(), &pl)) return 0;
+    info[0] = pl.tp ? 2 : 1;
+    info[1] = pl.nb;
+    info[2] = pl.G;
+    info[3] = pl.rpm;
+    info[4] = pl.ne_lds > 0 ? 1 : 0;
+    info[5] = (int)pl.lds;
+    return 0;
+}

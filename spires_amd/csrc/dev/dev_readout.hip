@@ -196,12 +196,19 @@ using namespace sdev;
 extern "C" {
 
 int spires_dev_ridge(spires_dev* g, size_t washout, const double* Y, double lambda, double* info_out) {
+    if (!g) return SPIRES_DEV_EINVAL;
+    return spires_dev_ridge_rows(g, washout, g->rec_rows, Y, lambda, info_out);
+}
+
+int spires_dev_ridge_rows(spires_dev* g, size_t row0, size_t row1, const double* Y, double lambda,
+                          double* info_out) {
     hipError_t herr = hipSuccess;
     auto H = [&](hipError_t e) { if (e != hipSuccess && herr == hipSuccess) herr = e; };
     if (!g || !Y) return SPIRES_DEV_EINVAL;
-    if (!g->rec_state || washout >= g->rec_rows) { set_error("no recorded states for ridge"); return SPIRES_DEV_EINVAL; }
+    if (!g->rec_state || row0 >= row1 || row1 > g->rec_rows) { set_error("no recorded states for ridge"); return SPIRES_DEV_EINVAL; }
+    const size_t washout = row0;
     const int n = g->n, D = g->dout;
-    const int T = (int)(g->rec_rows - washout);
+    const int T = (int)(row1 - row0);
     if (D == 0) return 0;
     rocblas_handle h = get_handle(g);
     if (!h) { set_error("rocblas_create_handle failed"); return SPIRES_DEV_EINTERNAL; }

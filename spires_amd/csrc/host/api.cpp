@@ -621,6 +621,12 @@ spires_status spires_ext_run_resident(spires_reservoir* r, size_t t0, size_t T, 
     return dev_status(spires_dev_run(r->dev, t0, T, flags));
 }
 
+spires_status spires_ext_fit_ridge_records(spires_reservoir* r, size_t row0, size_t row1, const double* Y,
+                                           double lambda, double* info) {
+    if (!r || !r->dev || !Y) return SPIRES_ERR_INVALID_ARG;
+    return dev_status(spires_dev_ridge_rows(r->dev, row0, row1, Y, lambda, info));
+}
+
 spires_status spires_ext_run_resident_batch(spires_reservoir* const* rs, int count, size_t t0, size_t T,
                                             unsigned flags) {
     if (!rs || count < 1) return SPIRES_ERR_INVALID_ARG;
@@ -630,6 +636,11 @@ spires_status spires_ext_run_resident_batch(spires_reservoir* const* rs, int cou
         devs[i] = rs[i]->dev;
     }
     return dev_status(spires_dev_run_batch(devs.data(), count, t0, T, flags));
+}
+
+spires_status spires_ext_kernel_plan(spires_reservoir* r, int* info) {
+    if (!r || !r->dev || !info) return SPIRES_ERR_INVALID_ARG;
+    return dev_status(spires_dev_plan_info(r->dev, info));
 }
 
 int spires_ext_batch_workgroups(spires_reservoir* const* rs, int count) {
