@@ -18,11 +18,18 @@ With --copy-calibration a plain device-to-device copy of 256 MiB runs first:
 its FETCH_SIZE / WRITE_SIZE under rocprofv3 --pmc calibrate the counters that
 give the step kernel's HBM traffic.
 
+With --agile it runs the config-4 hyperparameter search instead: the plain
+batched FP64 search over 512 trials at 18000 steps sets the target NRMSE, and
+successive halving and the surrogate-guided search are timed until they come
+within 2 % of it (value = median GPU-seconds over the surrogate-search seeds).
+
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--copy-calibration]
+                       [--agile [--agile-seeds S]]
 """
 import argparse
 import ctypes as C
 import json
+import math
 import os
 import sys
 import time
@@ -58,13 +65,6 @@ class HipEvents:
         ms = C.c_float()
         assert self.hip.hipEventElapsedTime(C.byref(ms), a, b) == 0
         return ms.value
-
-
-def algorithmic_bytes_per_neuron_step(n, nnz, L):
-    """Bytes one neuron-step must move in the FP64 exact step: the L-1 history
-    samples, the new ring sample, the CSR row (int32 col + f64 value per edge,
-    int64 row pointer), W_in, the spike byte and the state-record write."""
-    return 8 * (L - 1) + 8 + (nnz / n) * 12 + 8 + 8 + 1 + 8
 
 
 def measured_traffic_per_step():
@@ -105,6 +105,103 @@ def cpu_baseline(cfg, csr, W_in, u, budget_s=12.0):
                       f"OpenMP over neurons, {dt:.1f} s)"}
 
 
+def agile_cpu_baseline(budget_s=10.0):
+    """The oracle's C restatement (OpenMP over neurons) stepping one search
+    trial (N=2048 FLIF-GL, T_mem=544 -- the mean of the T_mem range -- ER
+    p=0.045, the median connection probability) for a bounded sample of steps;
+    the plain search's 512 x 18000-step runs are priced at that rate."""
+    import numpy as np
+    from oracle import pyoracle as po
+    from spires_amd import agile_search as ag
+    po.build()
+    th = np.array([1.0, 0.65, 1.0, 0.045, 544.0, 1e-5])
+    ev = ag.Evaluator.__new__(ag.Evaluator)
+    ev.n = 2048
+    rp, ci, val, W_in = ag.Evaluator._weights(ev, th, 1)
+    from spires_amd import workloads
+    params = workloads.flif_gl_params(alpha=th[1], T_mem=th[4])
+    o = po.OracleReservoir(po.make_config(2048, din=1, dout=1, gain=th[2], neuron="flif_gl", params=params),
+                           csr=(rp, ci, val), W_in=W_in)
+    u = workloads.mackey_glass(4000, seed=0).reshape(-1, 1)
+    L = int(th[4])
+    o.run_record(u[:L], L, states=False, raster=False)       # history ramp, untimed
+    done, chunk = 0, 25
+    t0 = time.perf_counter()
+    while time.perf_counter() - t0 < budget_s:
+        a = L + done % (len(u) - L - chunk)                    # the input series, cycled
+        o.run_record(u[a:a + chunk], chunk, states=True, raster=False)
+        done += chunk
+    dt = time.perf_counter() - t0
+    rate = 2048 * done / dt
+    cores = int(os.environ.get("OMP_NUM_THREADS", len(os.sched_getaffinity(0))))
+    plain_s = 512 * 18000 * 2048 / rate
+    return {"value": 3600.0 / (plain_s / 512), "unit": "trials/hour (18000-step trials, simulation only)",
+            "cores": cores, "kind": "port", "neuron_steps_per_s": rate,
+            "sample": f"{done} steady-state steps of one N=2048 T_mem=544 trial (oracle C restatement, "
+                      f"OpenMP over neurons, {dt:.1f} s); plain search priced at {plain_s:.0f} s"}
+
+
+def run_agile(args):
+    """Config-4 hyperparameter search: the plain batched FP64 search (all 512
+    trials at 18000 steps) sets the target; successive halving (no surrogate)
+    and the surrogate-guided search (several search seeds) are timed until a
+    top-rung validation NRMSE is within 2 % of it."""
+    import numpy as np
+    from spires_amd import agile_search as ag
+    rungs = (2000, 6000, 18000)
+    count = 512
+    th = ag.sample_trials(count, seed=2024)
+    seeds = np.arange(count) + 10000
+    log = (lambda m: print(m, file=sys.stderr, flush=True))
+    ev = ag.Evaluator(n=2048, f_max=rungs[-1], seed=0)
+    plain = ag.plain_search(ev, th, seeds, rungs[-1], log=log)
+    ev.close()
+    plain.pop("scores")
+    target = plain["best"]
+    ev = ag.Evaluator(n=2048, f_max=rungs[-1], seed=0)
+    halving = ag.halving_search(ev, th, seeds, rungs, target=target, log=log)
+    ev.close()
+    runs = []
+    for sd in range(1, args.agile_seeds + 1):
+        ev = ag.Evaluator(n=2048, f_max=rungs[-1], seed=0)
+        r = ag.surrogate_search(ev, rungs, target=target, seed=sd, log=log)
+        ev.close()
+        r.pop("history")
+        r["seed"] = sd
+        r["matched"] = r["best"] <= 1.02 * target
+        runs.append(r)
+    times = sorted(r["elapsed"] if r["matched"] else math.inf for r in runs)
+    med = float(np.median(times))
+    halving["matched"] = halving["best"] <= 1.02 * target
+    gbs = plain["alg_bytes"] / plain["run_s"] / 1e9
+    line = {
+        "metric": METRIC,
+        "value": med,
+        "unit": "GPU-s to matched val NRMSE (median over surrogate-search seeds)",
+        "n_gpus": 1,
+        "higher_is_better": False,
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "AGILE search: FLIF-GL N=2048, 512 trials, rungs 2000/6000/18000 steps, "
+                               "84-step-ahead Mackey-Glass, NRMSE; 6 hyperparameters; surrogate 7-64-64-1 FP32 "
+                               "Adam lr=1e-3", "target_nrmse": target, "target_tolerance": 0.02},
+        "plain_gpu_s": plain["elapsed"],
+        "speedup_median": plain["elapsed"] / med if med > 0 else None,
+        "matched": f"{sum(r['matched'] for r in runs)}/{len(runs)}",
+        "trials_per_hour_plain": 3600.0 * count / plain["elapsed"],
+        "plain": plain,
+        "halving": halving,
+        "surrogate": runs,
+        "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": gbs / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_persist_tp_batch (plain search launches, algorithmic bytes / launch time)"},
+    }
+    if not args.no_cpu:
+        line["cpu_baseline"] = agile_cpu_baseline()
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -113,6 +210,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first copy 256 MiB device-to-device (counter calibration for rocprofv3 --pmc)")
+    ap.add_argument("--agile", action="store_true",
+                    help="run the config-4 hyperparameter search comparison instead of the step benchmark")
+    ap.add_argument("--agile-seeds", type=int, default=6)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs one reservoir per device and has no "
@@ -125,6 +225,9 @@ def main():
 
     if not os.path.exists(sa.LIB_PATH):
         sa.build()
+    if args.agile:
+        run_agile(args)
+        return
     if args.copy_calibration:
         import torch
         nbytes = 256 << 20
@@ -157,7 +260,7 @@ def main():
     value = n * K / (gpu_ms * 1e-3)
     X, _, _ = r.read_records(K - 1, 1, states=True)
     nnz = len(csr[2])
-    bps = algorithmic_bytes_per_neuron_step(n, nnz, L)
+    bps = workloads.algorithmic_bytes_per_neuron_step(n, nnz, L)
     achieved = bps * n / (ms_per_step * 1e-3) / 1e9
     traffic_step, traffic_src = measured_traffic_per_step()
     line = {

@@ -82,6 +82,8 @@ int  spires_dev_upload_inputs(spires_dev *g, const double *u, size_t T);
 #define SPIRES_DEV_REC_STATE  1u   /* write V after each step into the state record */
 #define SPIRES_DEV_REC_RASTER 2u   /* write spikes (u8) into the raster record        */
 #define SPIRES_DEV_REC_OUTPUT 4u   /* write y = W_out V into the output record        */
+#define SPIRES_DEV_REC_APPEND 8u   /* record step t0+k at row t0+k (keeping rows below t0),
+                                      not at row k                                      */
 /* Record buffers are (re)allocated to hold T rows starting at row 0. */
 int  spires_dev_run(spires_dev *g, size_t t0, size_t T, unsigned flags);
 /* spires_dev_run for `count` exact-mode FLIF-GL reservoirs at once: one
@@ -120,6 +122,15 @@ int  spires_dev_ridge(spires_dev *g, size_t washout, const double *Y, double lam
 /* As spires_dev_ridge on the recorded rows [row0, row1) with Y[(row1-row0) x dout]. */
 int  spires_dev_ridge_rows(spires_dev *g, size_t row0, size_t row1, const double *Y, double lambda,
                            double *info_out);
+/* spires_dev_ridge_rows for `count` exact-mode reservoirs of the same N and
+ * Dout over the same row range, with one lambda each: the factorisations and
+ * solves run batched.  info_out[2r] is 0 (Cholesky), 1 (LU) or -1 (singular:
+ * that W_out is left unchanged), info_out[2r+1] the relative residual. */
+int  spires_dev_ridge_rows_batch(spires_dev *const *gs, int count, size_t row0, size_t row1,
+                                 const double *const *Y, const double *lambda, double *info_out);
+/* Readout outputs Y[rows x Dout] = X W_out' of recorded state rows
+ * [row0, row0+rows), computed on the device and copied to host `out`. */
+int  spires_dev_predict_records(spires_dev *g, size_t row0, size_t rows, double *out);
 /* Recursive least squares over T resident input steps with targets Y[T x dout]. */
 int  spires_dev_rls(spires_dev *g, const double *Y, size_t T, double delta, double lambda);
 /* Delta rule on the current state. */

@@ -52,6 +52,7 @@ spires_status spires_ext_run_record(spires_reservoir *r, const double *u, size_t
 #define SPIRES_EXT_REC_STATE  1u
 #define SPIRES_EXT_REC_RASTER 2u
 #define SPIRES_EXT_REC_OUTPUT 4u
+#define SPIRES_EXT_REC_APPEND 8u   /* record macro step t0+k at row t0+k, keeping rows [0, t0) */
 spires_status spires_ext_upload_inputs(spires_reservoir *r, const double *u, size_t T);
 spires_status spires_ext_reserve_records(spires_reservoir *r, size_t rows, unsigned flags);
 spires_status spires_ext_run_resident(spires_reservoir *r, size_t t0, size_t T, unsigned flags);
@@ -71,6 +72,17 @@ spires_status spires_ext_train_ridge_washout(spires_reservoir *r, const double *
  * spires_ext_train_ridge_washout. */
 spires_status spires_ext_fit_ridge_records(spires_reservoir *r, size_t row0, size_t row1, const double *Y,
                                            double lambda, double *info);
+/* spires_ext_fit_ridge_records for `count` reservoirs at once (one lambda and
+ * one Y[(row1-row0) x Dout] each); exact-mode reservoirs of equal N and Dout
+ * are factorised together.  info[2r] is 0 (Cholesky), 1 (LU) or -1 (singular
+ * system, that W_out unchanged); info[2r+1] the relative residual.  No
+ * counterpart in the reference. */
+/* Readout outputs of recorded state rows [row0, row0+rows) under the current
+ * W_out, into out[rows x Dout] (host).  No counterpart in the reference. */
+spires_status spires_ext_predict_records(spires_reservoir *r, size_t row0, size_t rows, double *out);
+spires_status spires_ext_fit_ridge_records_batch(spires_reservoir *const *rs, int count, size_t row0,
+                                                 size_t row1, const double *const *Y, const double *lambda,
+                                                 double *info);
 
 /* Batched runs: `count` reservoirs advance over macro steps [t0, t0+T) of
  * their own uploaded input series in one device launch, each into its own

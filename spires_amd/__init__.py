@@ -23,7 +23,7 @@ CONN = {"random": 0, "small_world": 1, "scale_free": 2}
 BACKEND = {"sparse": 0, "dense": 1}
 STATUS = {0: "SPIRES_OK", 1: "SPIRES_ERR_INVALID_ARG", 2: "SPIRES_ERR_ALLOC", 3: "SPIRES_ERR_INTERNAL"}
 
-REC_STATE, REC_RASTER, REC_OUTPUT = 1, 2, 4
+REC_STATE, REC_RASTER, REC_OUTPUT, REC_APPEND = 1, 2, 4, 8
 MODE_EXACT, MODE_FAST = 0, 1
 PATH_AUTO, PATH_PER_STEP, PATH_PERSISTENT, PATH_PERSISTENT_HBM, PATH_PERSISTENT_LDS = 0, 1, 2, 3, 4
 
@@ -106,6 +106,8 @@ SIGNATURES = {
     "spires_ext_fit_ridge_records": (C.c_int, [_P, _SZ, _SZ, _D, C.c_double, _D]),
     "spires_ext_batch_workgroups": (C.c_int, [C.POINTER(_P), C.c_int]),
     "spires_ext_kernel_plan": (C.c_int, [_P, C.POINTER(C.c_int)]),
+    "spires_ext_predict_records": (C.c_int, [_P, _SZ, _SZ, _D]),
+    "spires_ext_fit_ridge_records_batch": (C.c_int, [C.POINTER(_P), C.c_int, _SZ, _SZ, C.POINTER(_D), _D, _D]),
     "spires_ext_train_ridge_washout": (C.c_int, [_P, _D, _D, _SZ, _SZ, C.c_double, _D]),
     "spires_ext_set_mode": (C.c_int, [_P, C.c_int, C.c_int]),
     "spires_ext_set_kernel_path": (C.c_int, [_P, C.c_int]),
@@ -235,6 +237,20 @@ def run_resident_batch(reservoirs, t0, T, flags=None):
     flags = REC_STATE if flags is None else flags
     arr = (_P * len(reservoirs))(*[r.h for r in reservoirs])
     _check(lib().spires_ext_run_resident_batch(arr, len(reservoirs), t0, T, flags))
+
+
+def fit_ridge_records_batch(reservoirs, row0, row1, Ys, lams):
+    """Ridge W_out of each reservoir on its recorded rows [row0, row1) against
+    its own Y and lambda, factorised together (spires_ext_fit_ridge_records_batch).
+    Returns [count x 2]: (0 Cholesky / 1 LU / -1 singular, relative residual)."""
+    k = len(reservoirs)
+    arr = (_P * k)(*[r.h for r in reservoirs])
+    Ys = [_f64(Y).reshape(row1 - row0, -1) for Y in Ys]
+    yp = (_D * k)(*[_dp(Y) for Y in Ys])
+    lam = _f64(np.asarray(lams, dtype=np.float64).reshape(k))
+    info = np.zeros((k, 2))
+    _check(lib().spires_ext_fit_ridge_records_batch(arr, k, row0, row1, yp, _dp(lam), _dp(info)))
+    return info
 
 
 def batch_workgroups(reservoirs):
@@ -391,6 +407,12 @@ class Reservoir:
         info = np.zeros(2)
         _check(lib().spires_ext_fit_ridge_records(self.h, row0, row1, _dp(Y), lam, _dp(info)))
         return bool(info[0]), float(info[1])
+
+    def predict_records(self, row0, rows):
+        """Readout outputs [rows x Dout] of recorded state rows under the current W_out."""
+        y = np.empty((rows, self.Dout))
+        _check(lib().spires_ext_predict_records(self.h, row0, rows, _dp(y)))
+        return y
 
     def read_records(self, row0, rows, states=False, raster=False, outputs=False):
         X = np.empty((rows, self.N)) if states else None

@@ -3,37 +3,39 @@ workload with fidelity rungs and a learned surrogate).
 
 Task: 84-step-ahead forecasting of the Mackey-Glass series (tau=17, RK4
 dt=0.1, sampled every 1.0).  A trial is a FLIF-GL reservoir of N neurons with
-six hyperparameters:
+six hyperparameters
     rho    U[0.5, 1.5]       spectral radius of the recurrent weights
     alpha  U[0.3, 1.0]       fractional order
     gain   log-U[0.1, 10]    input gain
     conn   log-U[0.01, 0.2]  Erdos-Renyi connection probability
     T_mem  U[64, 1024]       history length (integer, dt = 1)
     lam    log-U[1e-8, 1e-2] ridge regulariser
-evaluated at a fidelity of F steps: the reservoir runs F steps (FP64 exact
-mode, several trials per device launch), the ridge readout is fitted on the
-recorded states of steps [F/10, 3F/4) and scored on [3F/4, F) by
-NRMSE = RMSE / std(target).
+and a seed for its random weights.  At a fidelity of F steps the reservoir
+has run F steps (FP64 exact mode, several trials per device launch), the
+ridge readout is fitted on the recorded states of steps [F/10, 3F/4) and
+scored on [3F/4, F) by NRMSE = RMSE / std(target).  A trial promoted to a
+higher rung continues its run (the records of the lower rung are kept), so
+its scores at every rung are those of one uninterrupted run.
 
 Recurrent weights are N(0,1) on an Erdos-Renyi graph, scaled by
-rho / sqrt(N * conn): by the circular law the spectral radius of such a
-matrix is sqrt(N * conn) to within a few percent at these sizes, which keeps
-trial construction cheap (the exact power iteration of the reference costs
-1000 sparse products per trial).  Input weights are U[-1, 1].
+rho / sqrt(N * conn): by the circular law the spectral radius of such a matrix
+is sqrt(N * conn) to within a few percent at these sizes, which keeps trial
+construction cheap (the reference's power iteration costs 1000 sparse
+products per trial).  Input weights are U[-1, 1].
 
-Searches (all report only scores measured by full reservoir runs):
-  plain_search      every trial at the top fidelity;
-  halving_search    successive halving over the rungs without a surrogate;
-  surrogate_search  the same rungs, with the MLP surrogate (spires_amd.Surrogate,
-                    trained on the GPU on every (hyperparameters, fidelity,
-                    log10 NRMSE) record) screening candidates before they are
-                    built and ranking promotions between rungs.
-The last two keep promoting candidates to the top rung, best-ranked first,
-until a target score is matched; `elapsed` is the wall time of the search on
-one GPU.
+Searches (every reported score is a full reservoir run at that fidelity):
+  plain_search      all trials at the top rung;
+  halving_search    successive halving over the rungs, no surrogate;
+  surrogate_search  the MLP surrogate (spires_amd.Surrogate, trained on the
+                    GPU on every (hyperparameters, fidelity, log10 NRMSE)
+                    record) picks which proposals to run and which trials to
+                    promote; proposals are global samples and Levy-flight
+                    moves around the best trials so far.
+`elapsed` is the wall time of a search on one GPU, host work included.
 """
 import math
 import time
+from concurrent.futures import ThreadPoolExecutor
 
 import numpy as np
 
@@ -42,6 +44,20 @@ from spires_amd import workloads
 
 NAMES = ("rho", "alpha", "gain", "conn", "T_mem", "lam")
 HORIZON = 84
+LO = np.array([0.5, 0.3, -1.0, -2.0, 64.0, -8.0])           # rho, alpha, log10 gain, log10 conn, T_mem, log10 lam
+HI = np.array([1.5, 1.0, 1.0, math.log10(0.2), 1024.0, -2.0])
+
+
+def to_unit(th):
+    """Hyperparameters -> [0, 1]^6 (log scales where sampled log-uniformly)."""
+    th = np.atleast_2d(th)
+    z = np.column_stack([th[:, 0], th[:, 1], np.log10(th[:, 2]), np.log10(th[:, 3]), th[:, 4], np.log10(th[:, 5])])
+    return (z - LO) / (HI - LO)
+
+
+def from_unit(z):
+    z = np.clip(np.atleast_2d(z), 0.0, 1.0) * (HI - LO) + LO
+    return np.column_stack([z[:, 0], z[:, 1], 10.0 ** z[:, 2], 10.0 ** z[:, 3], np.floor(z[:, 4]), 10.0 ** z[:, 5]])
 
 
 def sample_trials(count, seed):
@@ -57,43 +73,86 @@ def sample_trials(count, seed):
     return th
 
 
+def latin_hypercube(count, seed):
+    """[count x 6] hyperparameters from a Latin hypercube over the search
+    distributions: each axis is cut into `count` equal-probability strata and
+    every stratum holds one sample, so a small first wave still covers every
+    axis end to end."""
+    rng = np.random.default_rng(seed)
+    z = (np.argsort(rng.random((6, count)), axis=1).T + rng.random((count, 6))) / count
+    return from_unit(z)
+
+
 def encode(th, fidelity, f_min, f_max):
-    """Surrogate input: the six hyperparameters mapped to [0, 1] (log scales
-    where sampled log-uniformly) and log fidelity mapped to [0, 1]."""
-    th = np.atleast_2d(th)
-    z = np.empty((th.shape[0], 7), np.float32)
-    z[:, 0] = (th[:, 0] - 0.5) / 1.0
-    z[:, 1] = (th[:, 1] - 0.3) / 0.7
-    z[:, 2] = (np.log10(th[:, 2]) + 1.0) / 2.0
-    z[:, 3] = (np.log10(th[:, 3]) + 2.0) / (math.log10(0.2) + 2.0)
-    z[:, 4] = (th[:, 4] - 64.0) / 960.0
-    z[:, 5] = (np.log10(th[:, 5]) + 8.0) / 6.0
-    z[:, 6] = math.log(fidelity / f_min) / max(1e-12, math.log(f_max / f_min))
-    return z
+    """Surrogate input: the six hyperparameters on [0, 1] and log fidelity on [0, 1]."""
+    z = to_unit(th)
+    fz = math.log(fidelity / f_min) / max(1e-12, math.log(f_max / f_min))
+    return np.column_stack([z, np.full(len(z), fz)]).astype(np.float32)
+
+
+class Trial:
+    """One (hyperparameters, weight seed) candidate and its run so far."""
+    __slots__ = ("theta", "seed", "res", "future", "done", "scores", "nb", "bps")
+
+    def __init__(self, theta, seed):
+        self.theta = np.asarray(theta, dtype=np.float64)
+        self.seed = int(seed)
+        self.res = None
+        self.future = None
+        self.done = 0
+        self.scores = {}
+        self.nb = 0
+        self.bps = 0.0          # algorithmic bytes per neuron-step
 
 
 class Evaluator:
-    """Builds trials and scores them at a fidelity on the GPU."""
+    """Builds trials (on host threads, overlapping device work) and advances
+    them to a fidelity on the GPU: batched runs, batched ridge fits, readout
+    predictions on the device."""
 
-    def __init__(self, n=2048, f_max=18000, seed=0):
+    def __init__(self, n=2048, f_max=18000, seed=0, workers=4):
         self.n = n
-        self.series = workloads.mackey_glass(f_max + HORIZON + 1, seed=seed)
-        self.kernel_steps = 0          # trial-steps simulated
+        self.f_max = f_max
+        series = workloads.mackey_glass(f_max + HORIZON + 1, seed=seed)
+        self.u = np.ascontiguousarray(series[:f_max].reshape(f_max, 1))
+        self.y = np.ascontiguousarray(series[HORIZON:f_max + HORIZON])
+        self.pool = ThreadPoolExecutor(max_workers=workers)
+        self.trial_steps = 0           # reservoir steps simulated, all trials
         self.trials_built = 0
-        self.evaluations = 0
-        self.run_seconds = 0.0         # synchronous reservoir launches
-        self.fit_seconds = 0.0         # ridge fits and validation readback
-        self.build_seconds = 0.0       # host-side trial construction and upload
+        self.fits = 0
+        self.run_seconds = 0.0         # device launches (synchronous)
+        self.fit_seconds = 0.0         # ridge fits and validation predictions
+        self.wait_seconds = 0.0        # waiting for trial construction
+        self.alg_bytes = 0.0           # algorithmic HBM bytes of the simulated steps
+
+    def close(self):
+        self.pool.shutdown(wait=True)
+
+    @staticmethod
+    def split(F):
+        return max(50, F // 10), (3 * F) // 4
 
     def _weights(self, th, trial_seed):
+        """Erdos-Renyi edges by geometric gaps between successive edges of the
+        row-major n x n index space (no self loops), N(0,1) values."""
         rng = np.random.default_rng(trial_seed)
-        n = self.n
-        mask = rng.random((n, n)) < th[3]
-        np.fill_diagonal(mask, False)
-        rows, cols = np.nonzero(mask)
+        n, c = self.n, float(th[3])
+        total = n * n
+        inv = 1.0 / math.log1p(-c)
+
+        def gaps(k):   # Geometric(c) on {1, 2, ...} by inversion
+            return (np.log(1.0 - rng.random(k)) * inv).astype(np.int64) + 1
+
+        pos = np.cumsum(gaps(int(total * c * 1.1) + 64)) - 1
+        while pos[-1] < total:
+            pos = np.concatenate([pos, np.cumsum(gaps(int(total * c * 0.1) + 64)) + pos[-1]])
+        pos = pos[pos < total]
+        rows, cols = np.divmod(pos, n)
+        keep = rows != cols
+        rows, cols = rows[keep], cols[keep]
         rp = np.zeros(n + 1, np.int64)
         np.cumsum(np.bincount(rows, minlength=n), out=rp[1:])
-        val = rng.standard_normal(len(cols)) * (th[0] / math.sqrt(n * th[3]))
+        val = rng.standard_normal(len(cols), dtype=np.float32) * (th[0] / math.sqrt(n * th[3]))
         W_in = rng.uniform(-1.0, 1.0, (n, 1))
         return rp, cols.astype(np.int32), val, W_in
 
@@ -102,157 +161,275 @@ class Evaluator:
         cfg = sa.make_config(self.n, din=1, dout=1, gain=float(th[2]), conn=float(th[3]), neuron="flif_gl",
                              params=params)
         rp, ci, val, W_in = self._weights(th, trial_seed)
-        self.trials_built += 1
-        return sa.Reservoir(cfg, csr=(rp, ci, val), W_in=W_in)
+        r = sa.Reservoir(cfg, csr=(rp, ci, val), W_in=W_in)
+        r.upload_inputs(self.u)
+        r.bps = workloads.algorithmic_bytes_per_neuron_step(self.n, len(ci), int(th[4]))
+        return r
 
-    def _groups(self, res):
-        """Greedy groups of reservoirs that run together in one launch."""
-        groups, cur = [], []
-        for r in res:
-            if cur and sa.batch_workgroups(cur + [r]) == 0:
-                groups.append(cur)
-                cur = []
-            cur.append(r)
-        if cur:
-            groups.append(cur)
-        return groups
+    def prefetch(self, trials):
+        """Start building trials on the host threads."""
+        for t in trials:
+            if t.res is None and t.future is None:
+                t.future = self.pool.submit(self.build, t.theta, t.seed)
 
-    def evaluate(self, thetas, seeds, F):
-        """Validation NRMSE of each trial at fidelity F (FP64 exact, batched)."""
-        u = self.series[:F].reshape(F, 1)
-        y = self.series[HORIZON:F + HORIZON]
-        w, split = max(50, F // 10), (3 * F) // 4
-        tb = time.perf_counter()
-        res = [self.build(th, s) for th, s in zip(thetas, seeds)]
-        for r in res:
-            r.upload_inputs(u)
-            r.reserve_records(F, sa.REC_STATE)
-        tr = time.perf_counter()
-        self.build_seconds += tr - tb
-        # one launch per group of trials with the same neurons per workgroup
-        nbs = [r.kernel_plan()["nb"] for r in res]
-        order = sorted(range(len(res)), key=lambda k: (nbs[k], thetas[k][4]))
-        for grp in self._groups([res[k] for k in order]):
-            if len(grp) > 1:
-                sa.run_resident_batch(grp, 0, F, sa.REC_STATE)
-            else:
-                grp[0].run_resident(0, F, sa.REC_STATE)
-        tf = time.perf_counter()
-        self.run_seconds += tf - tr
-        out = np.empty(len(res))
-        yv = y[split:F]
-        sd = float(np.std(yv))
-        for k, (r, th) in enumerate(zip(res, thetas)):
-            try:
-                r.fit_ridge_records(w, split, y[w:split].reshape(-1, 1), float(th[5]))
-                X, _, _ = r.read_records(split, F - split, states=True)
-                pred = X @ r.W_out().reshape(-1)
-                e = float(np.sqrt(np.mean((pred - yv) ** 2)) / sd)
-                out[k] = e if np.isfinite(e) else 1e3
-            except RuntimeError:
-                out[k] = 1e3
-            r.close()
-        self.fit_seconds += time.perf_counter() - tf
-        self.kernel_steps += F * len(res)
-        self.evaluations += len(res)
+    def _ensure(self, trials):
+        self.prefetch(trials)
+        t0 = time.perf_counter()
+        for t in trials:
+            if t.res is None:
+                t.res = t.future.result()
+                t.future = None
+                t.nb = t.res.kernel_plan()["nb"]
+                t.bps = t.res.bps
+                self.trials_built += 1
+        self.wait_seconds += time.perf_counter() - t0
+
+    def release(self, trials):
+        """Free the device memory of trials that will not run further."""
+        for t in trials:
+            if t.future is not None:
+                t.res = t.future.result()
+                t.future = None
+            if t.res is not None:
+                t.res.close()
+                t.res = None
+
+    def _launch_groups(self, trials):
+        """Trials that start at the same step, grouped by neurons per
+        workgroup and packed greedily into launches that fit the device."""
+        by_key = {}
+        for t in trials:
+            by_key.setdefault((t.nb, t.done), []).append(t)
+        out = []
+        for (_, done), ts in sorted(by_key.items()):
+            ts.sort(key=lambda q: q.theta[4])
+            cur = []
+            for t in ts:
+                if cur and sa.batch_workgroups([q.res for q in cur + [t]]) == 0:
+                    out.append((done, cur))
+                    cur = []
+                cur.append(t)
+            if cur:
+                out.append((done, cur))
         return out
 
+    def advance(self, trials, F, fit_batch=16):
+        """Run each trial on to fidelity F and score it there.  Returns the
+        validation NRMSE of each trial (1e3 for a singular fit or a blow-up)."""
+        trials = list(trials)
+        self._ensure(trials)
+        t0 = time.perf_counter()
+        for done, grp in self._launch_groups([t for t in trials if t.done < F]):
+            steps = F - done
+            if len(grp) > 1:
+                sa.run_resident_batch([t.res for t in grp], done, steps, sa.REC_STATE | sa.REC_APPEND)
+            else:
+                grp[0].res.run_resident(done, steps, sa.REC_STATE | sa.REC_APPEND)
+            for t in grp:
+                t.done = F
+                self.alg_bytes += t.bps * self.n * steps
+            self.trial_steps += steps * len(grp)
+        t1 = time.perf_counter()
+        self.run_seconds += t1 - t0
+        w, sp = self.split(F)
+        yv = self.y[sp:F]
+        sd = float(np.std(yv))
+        ytr = np.ascontiguousarray(self.y[w:sp].reshape(-1, 1))
+        todo = [t for t in trials if F not in t.scores]
+        for a in range(0, len(todo), fit_batch):
+            part = todo[a:a + fit_batch]
+            info = sa.fit_ridge_records_batch([t.res for t in part], w, sp, [ytr] * len(part),
+                                              [t.theta[5] for t in part])
+            for t, inf in zip(part, info):
+                e = 1e3
+                if inf[0] >= 0:
+                    pred = t.res.predict_records(sp, F - sp)[:, 0]
+                    e = float(np.sqrt(np.mean((pred - yv) ** 2)) / sd)
+                    e = e if np.isfinite(e) else 1e3
+                t.scores[F] = min(e, 1e3)
+            self.fits += len(part)
+        self.fit_seconds += time.perf_counter() - t1
+        return np.array([t.scores[F] for t in trials])
 
-def plain_search(ev, thetas, seeds, F):
-    """Every trial at fidelity F, in batches."""
+
+def _stats(ev, t0):
+    return dict(elapsed=time.perf_counter() - t0, trials_built=ev.trials_built, fits=ev.fits,
+                trial_steps=ev.trial_steps, run_s=round(ev.run_seconds, 3), fit_s=round(ev.fit_seconds, 3),
+                wait_s=round(ev.wait_seconds, 3), alg_bytes=ev.alg_bytes)
+
+
+def plain_search(ev, thetas, seeds, F, batch=32, log=None):
+    """Every trial at fidelity F, `batch` trials at a time (the next batch is
+    built on the host while the current one runs)."""
     t0 = time.perf_counter()
-    scores = np.empty(len(thetas))
-    step = 32
-    for a in range(0, len(thetas), step):
-        scores[a:a + step] = ev.evaluate(thetas[a:a + step], seeds[a:a + step], F)
-    return {"best": float(scores.min()), "best_trial": int(scores.argmin()), "scores": scores,
-            "elapsed": time.perf_counter() - t0}
+    trials = [Trial(th, s) for th, s in zip(thetas, seeds)]
+    scores = np.empty(len(trials))
+    for a in range(0, len(trials), batch):
+        part = trials[a:a + batch]
+        ev.prefetch(trials[a + batch:a + 2 * batch])
+        scores[a:a + batch] = ev.advance(part, F)
+        ev.release(part)
+        if log:
+            log(f"plain {a + len(part)}/{len(trials)} best {scores[:a + len(part)].min():.4f} "
+                f"t {time.perf_counter() - t0:.1f}s")
+    out = _stats(ev, t0)
+    out.update(best=float(scores.min()), best_trial=int(scores.argmin()), scores=scores)
+    return out
 
 
-def _promote_until(ev, thetas, seeds, ranked, F, target, best, log, t0, chunk=4):
-    """Run trials `ranked` (best first) at the top fidelity, `chunk` per launch,
-    until the best measured score is within 2 % of target."""
-    done = 0
-    while done < len(ranked) and not (target is not None and best <= 1.02 * target):
-        idx = ranked[done:done + chunk]
-        sc = ev.evaluate(thetas[idx], seeds[idx], F)
-        for k, s in zip(idx, sc):
-            log.append((int(k), F, float(s)))
-        best = min(best, float(sc.min()))
-        done += len(idx)
-        if target is None and done >= chunk:
+def _best_at(trials, F):
+    s = [t.scores[F] for t in trials if F in t.scores]
+    return min(s) if s else math.inf
+
+
+def halving_search(ev, thetas, seeds, rungs, target=None, eta=3, batch=32, log=None):
+    """Successive halving without a surrogate: every trial at the first rung,
+    the best 1/eta of each rung continue to the next; at the top rung trials
+    run best-first, four at a time, until one is within 2 % of target."""
+    t0 = time.perf_counter()
+    alive = [Trial(th, s) for th, s in zip(thetas, seeds)]
+    for ri, F in enumerate(rungs):
+        top = ri == len(rungs) - 1
+        step = 4 if (top and target is not None) else batch
+        for a in range(0, len(alive), step):
+            part = alive[a:a + step]
+            ev.prefetch(alive[a + step:a + 2 * step])
+            ev.advance(part, F)
+            if top and target is not None and _best_at(alive, F) <= 1.02 * target:
+                break
+        if log:
+            log(f"halving rung {F}: {len(alive)} trials, best {_best_at(alive, F):.4f} "
+                f"t {time.perf_counter() - t0:.1f}s")
+        if top:
             break
-    return best, time.perf_counter() - t0
+        alive.sort(key=lambda t: t.scores[F])
+        keep = max(1, len(alive) // eta)
+        ev.release(alive[keep:])
+        alive = alive[:keep]
+    best = _best_at(alive, rungs[-1])
+    ev.release(alive)
+    out = _stats(ev, t0)
+    out.update(best=best)
+    return out
 
 
-def halving_search(ev, thetas, seeds, rungs, target=None, eta=3):
-    """Successive halving without a surrogate: every trial at rung 0, the best
-    1/eta move up a rung; top-rung candidates in rung order until target."""
+def _levy_steps(rng, count, dim, mu=1.3, lo=1e-3, hi=0.3):
+    """Step lengths from a power law p(s) ~ s^-mu truncated to [lo, hi]
+    (inverse transform), times random unit directions."""
+    a = 1.0 - mu
+    u = rng.random(count)
+    s = (lo ** a + u * (hi ** a - lo ** a)) ** (1.0 / a)
+    d = rng.standard_normal((count, dim))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    return d * s[:, None]
+
+
+def surrogate_search(ev, rungs, target=None, budget=512, warm=96, wave=32, promote=8, pool=2048, seed=1,
+                     epochs=3000, round_epochs=500, lr=1e-3, reseed=4, explore=8, top_weight=4, clip=2.0,
+                     log=None):
+    """Surrogate-guided search with the fidelity rungs.
+
+    Every trial first runs to the middle rung (the lowest rung is a checkpoint
+    of the same run and is recorded too).  The surrogate is trained on all
+    (hyperparameters, fidelity, log10 NRMSE) records, the NRMSE clipped at
+    `clip` (a diverged trial's size carries no ranking information) and each
+    top-rung record counted `top_weight` times; the network is trained once
+    for `epochs` and then continued for `round_epochs` per round.  Of each
+    wave, half of the `promote` trials that continue to the top rung are the
+    best by measured middle-rung score and half the best by the surrogate's
+    top-rung prediction; the others are pruned.  The first wave holds `warm`
+    trials from a Latin hypercube; each later wave holds `wave` proposals:
+      - `reseed` fresh weight seeds for the two best hyperparameter sets so far
+        (a trial's score varies strongly with its weight draw);
+      - `explore` global samples, half unscreened and half the surrogate's
+        best-ranked of `pool`;
+      - the rest from its ranking of `pool` Levy-flight moves around the eight
+        best top-rung trials.
+    Every reported score comes from a full run; the search stops when a
+    top-rung score is within 2 % of target or after `budget` trials."""
     t0 = time.perf_counter()
-    log = []
-    alive = np.arange(len(thetas))
-    best = math.inf
-    for ri, F in enumerate(rungs[:-1]):
-        sc = np.empty(len(alive))
-        for a in range(0, len(alive), 32):
-            sc[a:a + 32] = ev.evaluate(thetas[alive[a:a + 32]], seeds[alive[a:a + 32]], F)
-        for k, s in zip(alive, sc):
-            log.append((int(k), F, float(s)))
-        alive = alive[np.argsort(sc)]
-        alive = alive[:max(1, len(alive) // eta)] if ri < len(rungs) - 2 else alive
-    best, el = _promote_until(ev, thetas, seeds, alive, rungs[-1], target, best, log, t0)
-    return {"best": best, "elapsed": el, "log": log}
-
-
-def surrogate_search(ev, thetas, seeds, rungs, target=None, warm=64, keep=0.25, promote=24, seed=1,
-                     epochs=400, lr=1e-3):
-    """Rungs as halving_search, with the surrogate screening candidates before
-    their first run and ranking promotions by its top-fidelity prediction."""
-    t0 = time.perf_counter()
+    rng = np.random.default_rng(seed)
     f_min, f_max = rungs[0], rungs[-1]
+    lower = list(rungs[:-1])
+    mid = lower[-1]
     net = sa.Surrogate(7, 64, seed=seed)
-    X, Y, log = [], [], []
+    X, Y = [], []
+    done = []           # (theta, top-rung score)
+    history = []        # (theta, seed, {fidelity: score}) of every trial
+    best = math.inf
+    ymax = math.log10(clip)
+    train_s = 0.0
 
-    def record(idx, F, sc):
-        for k, s in zip(idx, sc):
-            X.append(encode(thetas[k], F, f_min, f_max)[0])
-            Y.append(math.log10(max(float(s), 1e-6)))
-            log.append((int(k), F, float(s)))
+    def record(ts, fids):
+        for t in ts:
+            for F in fids:
+                if F in t.scores:
+                    for _ in range(top_weight if F == f_max else 1):
+                        X.append(encode(t.theta, F, f_min, f_max)[0])
+                        Y.append(min(math.log10(max(t.scores[F], 1e-6)), ymax))
 
-    def train():
-        net.train(np.array(X), np.array(Y, np.float32), epochs=epochs, lr=lr)
+    def train(ep):
+        nonlocal train_s
+        a = time.perf_counter()
+        net.train(np.array(X), np.array(Y, np.float32), epochs=ep, lr=lr)
+        train_s += time.perf_counter() - a
 
-    def predict(idx, F):
-        return net.predict(encode(thetas[idx], F, f_min, f_max))
+    def seeds(k):
+        return rng.integers(1 << 31, size=k)
 
-    # rung 0 on a warm-up wave, then screen the rest by predicted rung-0 score
-    first = np.arange(min(warm, len(thetas)))
-    sc = np.concatenate([ev.evaluate(thetas[first[a:a + 32]], seeds[first[a:a + 32]], f_min)
-                         for a in range(0, len(first), 32)])
-    record(first, f_min, sc)
-    train()
-    rest = np.arange(len(first), len(thetas))
-    if len(rest):
-        pr = predict(rest, f_min)
-        chosen = rest[np.argsort(pr)[:max(1, int(keep * len(rest)))]]
-        sc = np.concatenate([ev.evaluate(thetas[chosen[a:a + 32]], seeds[chosen[a:a + 32]], f_min)
-                             for a in range(0, len(chosen), 32)])
-        record(chosen, f_min, sc)
-        train()
-    seen = np.array(sorted({k for k, _, _ in log}))
-    # middle rungs: promote by predicted top-fidelity score
-    for F in rungs[1:-1]:
-        pred = predict(seen, f_max)
-        up = seen[np.argsort(pred)[:promote]]
-        sc = np.concatenate([ev.evaluate(thetas[up[a:a + 32]], seeds[up[a:a + 32]], F)
-                             for a in range(0, len(up), 32)])
-        record(up, F, sc)
-        train()
-        seen = up
-    ranked = seen[np.argsort(predict(seen, f_max))]
-    best, el = _promote_until(ev, thetas, seeds, ranked, f_max, target, math.inf, log, t0)
+    def rank(v):
+        return np.argsort(np.argsort(v))
+
+    def run_wave(new, ep):
+        nonlocal best
+        for a in range(0, len(new), 32):
+            part = new[a:a + 32]
+            ev.prefetch(new[a + 32:a + 64])
+            for F in lower:
+                ev.advance(part, F)
+        record(new, lower)
+        train(ep)
+        pred = net.predict(encode(np.array([t.theta for t in new]), f_max, f_min, f_max))
+        meas = np.array([t.scores[mid] for t in new])
+        chosen = list(np.argsort(meas, kind="stable")[:promote // 2])
+        chosen += [i for i in np.argsort(pred, kind="stable") if i not in chosen][:promote - len(chosen)]
+        up = [new[i] for i in chosen]
+        ev.release([t for t in new if t not in up])
+        ev.advance(up, f_max)
+        record(up, [f_max])
+        done.extend((t.theta, t.scores[f_max]) for t in up)
+        history.extend((t.theta.tolist(), t.seed, dict(t.scores)) for t in new)
+        best = min(best, _best_at(up, f_max))
+        ev.release(up)
+
+    count = warm
+    run_wave([Trial(th, s) for th, s in zip(latin_hypercube(warm, int(rng.integers(1 << 31))), seeds(warm))], epochs)
+    rounds = 0
+    if log:
+        log(f"surrogate warm-up: {count} trials, best {best:.4f} t {time.perf_counter() - t0:.1f}s "
+            f"(training {train_s:.2f}s)")
+    while not (target is not None and best <= 1.02 * target) and count < budget:
+        rounds += 1
+        train(round_epochs)
+        inc = sorted(done, key=lambda d: d[1])
+        pick = [inc[q % min(2, len(inc))][0] for q in range(reseed)]
+        glob = sample_trials(pool, int(rng.integers(1 << 31)))
+        pg = net.predict(encode(glob, f_max, f_min, f_max))
+        pick += list(glob[:explore // 2])                                   # unscreened
+        pick += [glob[i] for i in np.argsort(pg) if i >= explore // 2][:explore - explore // 2]
+        base = to_unit(np.array([d[0] for d in inc[:8]]))
+        loc = from_unit(base[rng.integers(len(base), size=pool)] + _levy_steps(rng, pool, 6, hi=0.15))
+        pl = net.predict(encode(loc, f_max, f_min, f_max))
+        pick += [loc[i] for i in np.argsort(pl)[:max(0, wave - len(pick))]]
+        run_wave([Trial(th, s) for th, s in zip(pick, seeds(len(pick)))], round_epochs)
+        count += len(pick)
+        if log:
+            log(f"surrogate round {rounds}: {count} trials, best {best:.4f} t {time.perf_counter() - t0:.1f}s "
+                f"(training {train_s:.2f}s)")
     net.close()
-    return {"best": best, "elapsed": el, "log": log}
+    out = _stats(ev, t0)
+    out.update(best=best, rounds=rounds, trials=count, train_s=round(train_s, 3), history=history)
+    return out
 
 
 def spearman(a, b):

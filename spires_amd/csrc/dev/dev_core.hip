@@ -467,23 +467,37 @@ int launch_output(spires_dev* g, double* y_dev) {
     return 0;
 }
 
-static int ensure_records(spires_dev* g, size_t rows, unsigned flags) {
+// Record buffers of at least `rows` rows; when one grows, its first `keep`
+// rows are carried over (appending runs).
+template <typename T>
+static int grow_record(spires_dev* g, T** buf, size_t* cap, size_t need, size_t keep_elems) {
+    if (need <= *cap) return 0;
+    T* nb = nullptr;
+    int rc = dalloc(&nb, need);
+    if (rc) return rc;
+    if (*buf && keep_elems) {
+        hipError_t e = hipMemcpyAsync(nb, *buf, keep_elems * sizeof(T), hipMemcpyDeviceToDevice, g->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(g->stream);
+        if (e != hipSuccess) { dfree(nb); return hip_fail(e, "record copy", __LINE__); }
+    }
+    dfree(*buf);
+    *buf = nb;
+    *cap = need;
+    return 0;
+}
+
+static int ensure_records(spires_dev* g, size_t rows, unsigned flags, size_t keep = 0) {
     int rc;
-    if ((flags & SPIRES_DEV_REC_STATE) && rows * g->n > g->rec_cap_s) {
-        dfree(g->rec_state);
-        if ((rc = dalloc(&g->rec_state, rows * g->n))) { g->rec_cap_s = 0; return rc; }
-        g->rec_cap_s = rows * g->n;
-    }
-    if ((flags & SPIRES_DEV_REC_RASTER) && rows * g->n > g->rec_cap_r) {
-        dfree(g->rec_raster);
-        if ((rc = dalloc(&g->rec_raster, rows * g->n))) { g->rec_cap_r = 0; return rc; }
-        g->rec_cap_r = rows * g->n;
-    }
-    if ((flags & SPIRES_DEV_REC_OUTPUT) && rows * g->dout > g->rec_cap_o) {
-        dfree(g->rec_out);
-        if ((rc = dalloc(&g->rec_out, rows * (size_t)g->dout))) { g->rec_cap_o = 0; return rc; }
-        g->rec_cap_o = rows * g->dout;
-    }
+    keep = std::min(keep, g->rec_rows);
+    if ((flags & SPIRES_DEV_REC_STATE) &&
+        (rc = grow_record(g, &g->rec_state, &g->rec_cap_s, rows * g->n, keep * g->n)))
+        return rc;
+    if ((flags & SPIRES_DEV_REC_RASTER) &&
+        (rc = grow_record(g, &g->rec_raster, &g->rec_cap_r, rows * g->n, keep * g->n)))
+        return rc;
+    if ((flags & SPIRES_DEV_REC_OUTPUT) &&
+        (rc = grow_record(g, &g->rec_out, &g->rec_cap_o, rows * (size_t)g->dout, keep * (size_t)g->dout)))
+        return rc;
     g->rec_rows = rows;
     return 0;
 }
@@ -652,17 +666,21 @@ int spires_dev_upload_inputs(spires_dev* g, const double* u, size_t T) {
 int spires_dev_run(spires_dev* g, size_t t0, size_t T, unsigned flags) {
     if (!g) return SPIRES_DEV_EINVAL;
     if (t0 + T > g->u_T && g->din > 0) { set_error("run beyond uploaded inputs"); return SPIRES_DEV_EINVAL; }
+    const size_t base = (flags & SPIRES_DEV_REC_APPEND) ? t0 : 0;   // first record row of this run
+    const size_t bn = base * (size_t)g->n;
     if (g->mode == SPIRES_DEV_MODE_FAST) {
         if (!fast_supported(g)) { set_error("fast mode supports FLIF-GL with SIMPLE/PSC sparse synapses only"); return SPIRES_DEV_EINVAL; }
         unsigned needf = flags | ((flags & SPIRES_DEV_REC_OUTPUT) ? SPIRES_DEV_REC_STATE : 0u);
-        int rcf = ensure_records(g, T, needf);
+        int rcf = ensure_records(g, base + T, needf, base);
         if (rcf) return rcf;
         if (T == 0) return 0;
         const double* u = g->din > 0 ? g->u_series : nullptr;
-        if ((rcf = launch_fast(g, u, (long long)t0, (long long)T, (needf & SPIRES_DEV_REC_STATE) ? g->rec_state : nullptr,
-                               (flags & SPIRES_DEV_REC_RASTER) ? g->rec_raster : nullptr)))
+        if ((rcf = launch_fast(g, u, (long long)t0, (long long)T,
+                               (needf & SPIRES_DEV_REC_STATE) ? g->rec_state + bn : nullptr,
+                               (flags & SPIRES_DEV_REC_RASTER) ? g->rec_raster + bn : nullptr)))
             return rcf;
-        if (flags & SPIRES_DEV_REC_OUTPUT) return outputs_from_states(g, g->rec_state, T, g->rec_out);
+        if (flags & SPIRES_DEV_REC_OUTPUT)
+            return outputs_from_states(g, g->rec_state + bn, T, g->rec_out + base * g->dout);
         return 0;
     }
     if (!g->ring64_valid) {
@@ -675,23 +693,24 @@ int spires_dev_run(spires_dev* g, size_t t0, size_t T, unsigned flags) {
     if (g->path >= 2 && !persist) { set_error("persistent kernel cannot run this reservoir"); return SPIRES_DEV_EINVAL; }
     unsigned need = flags;
     if (persist && (flags & SPIRES_DEV_REC_OUTPUT)) need |= SPIRES_DEV_REC_STATE;   // outputs from the state record
-    int rc = ensure_records(g, T, need);
+    int rc = ensure_records(g, base + T, need, base);
     if (rc) return rc;
     if (persist) {
         const double* u = g->din > 0 ? g->u_series : nullptr;
-        double* rv = (need & SPIRES_DEV_REC_STATE) ? g->rec_state : nullptr;
-        uint8_t* rs = (flags & SPIRES_DEV_REC_RASTER) ? g->rec_raster : nullptr;
+        double* rv = (need & SPIRES_DEV_REC_STATE) ? g->rec_state + bn : nullptr;
+        uint8_t* rs = (flags & SPIRES_DEV_REC_RASTER) ? g->rec_raster + bn : nullptr;
         if ((rc = launch_persist(g, u, (long long)t0, (long long)T, rv, rs))) return rc;
-        if (flags & SPIRES_DEV_REC_OUTPUT) return outputs_from_states(g, g->rec_state, T, g->rec_out);
+        if (flags & SPIRES_DEV_REC_OUTPUT)
+            return outputs_from_states(g, g->rec_state + bn, T, g->rec_out + base * g->dout);
         return 0;
     }
     for (size_t t = 0; t < T; ++t) {
         const double* u = g->din > 0 ? g->u_series + (t0 + t) * g->din : nullptr;
-        double* rv = (flags & SPIRES_DEV_REC_STATE) ? g->rec_state + t * g->n : nullptr;
-        uint8_t* rs = (flags & SPIRES_DEV_REC_RASTER) ? g->rec_raster + t * g->n : nullptr;
+        double* rv = (flags & SPIRES_DEV_REC_STATE) ? g->rec_state + (base + t) * g->n : nullptr;
+        uint8_t* rs = (flags & SPIRES_DEV_REC_RASTER) ? g->rec_raster + (base + t) * g->n : nullptr;
         if ((rc = launch_step(g, u, nullptr, rv, rs))) return rc;
         if (flags & SPIRES_DEV_REC_OUTPUT)
-            if ((rc = launch_output(g, g->rec_out + t * g->dout))) return rc;
+            if ((rc = launch_output(g, g->rec_out + (base + t) * g->dout))) return rc;
     }
     return 0;
 }
@@ -701,6 +720,7 @@ int spires_dev_run_batch(spires_dev* const* gs, int count, size_t t0, size_t T, 
     std::vector<double*> rv(count);
     std::vector<uint8_t*> rs(count);
     unsigned need = flags | ((flags & SPIRES_DEV_REC_OUTPUT) ? SPIRES_DEV_REC_STATE : 0u);
+    const size_t base = (flags & SPIRES_DEV_REC_APPEND) ? t0 : 0;
     for (int r = 0; r < count; ++r) {
         spires_dev* g = gs[r];
         if (!g) return SPIRES_DEV_EINVAL;
@@ -712,17 +732,17 @@ int spires_dev_run_batch(spires_dev* const* gs, int count, size_t t0, size_t T, 
             g->ring64_valid = 1;
         }
         g->ring32_valid = 0;
-        int rc = ensure_records(g, T, need);
+        int rc = ensure_records(g, base + T, need, base);
         if (rc) return rc;
-        rv[r] = (need & SPIRES_DEV_REC_STATE) ? g->rec_state : nullptr;
-        rs[r] = (flags & SPIRES_DEV_REC_RASTER) ? g->rec_raster : nullptr;
+        rv[r] = (need & SPIRES_DEV_REC_STATE) ? g->rec_state + base * (size_t)g->n : nullptr;
+        rs[r] = (flags & SPIRES_DEV_REC_RASTER) ? g->rec_raster + base * (size_t)g->n : nullptr;
     }
     if (T == 0) return 0;
     int rc = launch_persist_batch(gs, count, (long long)t0, (long long)T, rv.data(), rs.data());
     if (rc) return rc;
     if (flags & SPIRES_DEV_REC_OUTPUT)
         for (int r = 0; r < count; ++r)
-            if ((rc = outputs_from_states(gs[r], gs[r]->rec_state, T, gs[r]->rec_out))) return rc;
+            if ((rc = outputs_from_states(gs[r], rv[r], T, gs[r]->rec_out + base * gs[r]->dout))) return rc;
     return 0;
 }
 

@@ -5,13 +5,14 @@
 //
 // Parameters live in one device buffer, in this order:
 //   W1 [H x in] (row-major), b1 [H], W2 [H x H], b2 [H], W3 [H], b3 [1].
-// One workgroup of kMlpThreads threads processes the whole batch in slices of
-// kMlpSlice samples, so every gradient is accumulated in a fixed order (no
-// atomics): the results are deterministic run to run.  The batch is small
-// (a few hundred trial records), so one workgroup is plenty; the kernels are
-// bound by the barriers between layers, not by arithmetic.
+// Gradients are data parallel: workgroup w copies the parameters into LDS and
+// takes the batch's slices of kMlpSlice samples w, w + G, w + 2G, ... in that
+// order, summing its per-parameter partial gradient in registers; a second
+// kernel adds the G partials in workgroup order and applies Adam.  Every sum
+// has a fixed order (no atomics), so results are deterministic run to run.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -22,9 +23,11 @@
 namespace {
 
 constexpr int kMlpThreads = 256;
-constexpr int kMlpSlice = 32;        // samples per pass through the layers
+constexpr int kMlpSlice = 8;         // samples per pass through the layers
 constexpr int kMaxIn = 16;
 constexpr int kMaxHidden = 64;
+constexpr int kMaxParams = kMaxHidden * kMaxIn + kMaxHidden + kMaxHidden * kMaxHidden + 2 * kMaxHidden + 1;
+constexpr int kMaxGroups = 128;      // workgroups of one gradient pass
 
 struct MlpShape {
     int in, H;
@@ -45,13 +48,13 @@ __host__ __device__ inline MlpShape mlp_shape(int in, int H) {
 }
 
 // Forward pass of one slice (samples s0 .. s0+ns-1) into LDS: h1, h2 [slice][H]
-// and y [slice].  Thread t computes hidden unit t % H of samples t / H, t / H + T/H, ...
-__device__ void mlp_forward_slice(const MlpShape sh, const float* __restrict__ p, const float* __restrict__ X,
+// and y [slice], with the parameters w already in LDS.
+__device__ void mlp_forward_slice(const MlpShape sh, const float* __restrict__ w, const float* __restrict__ X,
                                   int s0, int ns, float* h1, float* h2, float* y) {
     const int tid = threadIdx.x, H = sh.H;
-    const float* W1 = p + sh.oW1; const float* b1 = p + sh.ob1;
-    const float* W2 = p + sh.oW2; const float* b2 = p + sh.ob2;
-    const float* W3 = p + sh.oW3;
+    const float* W1 = w + sh.oW1; const float* b1 = w + sh.ob1;
+    const float* W2 = w + sh.oW2; const float* b2 = w + sh.ob2;
+    const float* W3 = w + sh.oW3;
     for (int e = tid; e < ns * H; e += kMlpThreads) {
         const int s = e / H, j = e % H;
         float z = b1[j];
@@ -67,50 +70,55 @@ __device__ void mlp_forward_slice(const MlpShape sh, const float* __restrict__ p
     }
     __syncthreads();
     for (int s = tid; s < ns; s += kMlpThreads) {
-        float z = p[sh.ob3];
+        float z = w[sh.ob3];
         for (int j = 0; j < H; ++j) z += W3[j] * h2[s * kMaxHidden + j];
         y[s] = z;
     }
     __syncthreads();
 }
 
+__device__ void mlp_load_params(const MlpShape sh, const float* __restrict__ p, float* w) {
+    for (int i = threadIdx.x; i < sh.count; i += kMlpThreads) w[i] = p[i];
+    __syncthreads();
+}
+
 __global__ void __launch_bounds__(kMlpThreads) k_mlp_predict(MlpShape sh, const float* __restrict__ p,
                                                              const float* __restrict__ X, int B, float* out) {
+    __shared__ float w[kMaxParams];
     __shared__ float h1[kMlpSlice * kMaxHidden], h2[kMlpSlice * kMaxHidden], y[kMlpSlice];
-    for (int s0 = 0; s0 < B; s0 += kMlpSlice) {
+    mlp_load_params(sh, p, w);
+    for (int s0 = blockIdx.x * kMlpSlice; s0 < B; s0 += gridDim.x * kMlpSlice) {
         const int ns = min(kMlpSlice, B - s0);
-        mlp_forward_slice(sh, p, X, s0, ns, h1, h2, y);
+        mlp_forward_slice(sh, w, X, s0, ns, h1, h2, y);
         for (int s = threadIdx.x; s < ns; s += kMlpThreads) out[s0 + s] = y[s];
         __syncthreads();
     }
 }
 
-// Gradient of L = (1/B) sum_s (y_s - t_s)^2 with respect to all parameters,
-// written to g [count]; loss to *loss.  Thread t owns parameter t, t + T, ...
-// of the gradient and accumulates it over the slices in sample order.
-__global__ void __launch_bounds__(kMlpThreads) k_mlp_grad(MlpShape sh, const float* __restrict__ p,
-                                                          const float* __restrict__ X, const float* __restrict__ T,
-                                                          int B, float* __restrict__ g, float* loss) {
+// Partial gradient of workgroup blockIdx.x over its slices: gpart[blockIdx.x][count],
+// and its partial sum of squared errors lpart[blockIdx.x].
+__global__ void __launch_bounds__(kMlpThreads) k_mlp_grad_part(MlpShape sh, const float* __restrict__ p,
+                                                               const float* __restrict__ X,
+                                                               const float* __restrict__ T, int B,
+                                                               float* __restrict__ gpart, float* __restrict__ lpart) {
+    __shared__ float w[kMaxParams];
     __shared__ float h1[kMlpSlice * kMaxHidden], h2[kMlpSlice * kMaxHidden], y[kMlpSlice];
     __shared__ float dz2[kMlpSlice * kMaxHidden], dz1[kMlpSlice * kMaxHidden], dy[kMlpSlice];
     __shared__ float xs[kMlpSlice * kMaxIn];
     const int tid = threadIdx.x, H = sh.H, in = sh.in;
-    const float* W2 = p + sh.oW2; const float* W3 = p + sh.oW3;
-    constexpr int kPerThread = (kMaxHidden * kMaxIn + kMaxHidden + kMaxHidden * kMaxHidden + 2 * kMaxHidden + 1 +
-                                kMlpThreads - 1) / kMlpThreads;
+    mlp_load_params(sh, p, w);
+    const float* W2 = w + sh.oW2; const float* W3 = w + sh.oW3;
+    constexpr int kPerThread = (kMaxParams + kMlpThreads - 1) / kMlpThreads;
     float acc[kPerThread];
 #pragma unroll
     for (int q = 0; q < kPerThread; ++q) acc[q] = 0.0f;
     float lsum = 0.0f;
     const float inv_b = 1.0f / (float)B;
-    for (int s0 = 0; s0 < B; s0 += kMlpSlice) {
+    for (int s0 = blockIdx.x * kMlpSlice; s0 < B; s0 += gridDim.x * kMlpSlice) {
         const int ns = min(kMlpSlice, B - s0);
-        mlp_forward_slice(sh, p, X, s0, ns, h1, h2, y);
+        mlp_forward_slice(sh, w, X, s0, ns, h1, h2, y);
         for (int e = tid; e < ns * in; e += kMlpThreads) xs[(e / in) * kMaxIn + e % in] = X[(size_t)s0 * in + e];
-        for (int s = tid; s < ns; s += kMlpThreads) {
-            const float d = y[s] - T[s0 + s];
-            dy[s] = 2.0f * d * inv_b;
-        }
+        for (int s = tid; s < ns; s += kMlpThreads) dy[s] = 2.0f * (y[s] - T[s0 + s]) * inv_b;
         if (tid == 0)
             for (int s = 0; s < ns; ++s) { const float d = y[s] - T[s0 + s]; lsum += d * d; }
         __syncthreads();
@@ -156,26 +164,38 @@ __global__ void __launch_bounds__(kMlpThreads) k_mlp_grad(MlpShape sh, const flo
         }
         __syncthreads();
     }
+    float* gp = gpart + (size_t)blockIdx.x * sh.count;
 #pragma unroll
     for (int q = 0; q < kPerThread; ++q) {
         const int idx = tid + q * kMlpThreads;
-        if (idx < sh.count) g[idx] = acc[q];
+        if (idx < sh.count) gp[idx] = acc[q];
     }
-    if (tid == 0) *loss = lsum * inv_b;
+    if (tid == 0) lpart[blockIdx.x] = lsum;
 }
 
-// Adam (Kingma & Ba 2015) with bias correction; step is the 1-based update count.
-__global__ void k_mlp_adam(float* __restrict__ p, const float* __restrict__ g, float* __restrict__ m,
-                           float* __restrict__ v, int count, float lr, float beta1, float beta2, float eps,
-                           float bc1, float bc2) {
+// g = sum of the G partial gradients in workgroup order; loss = sum of the
+// partial squared errors / B.  With adam set, also one Adam step on p.
+__global__ void k_mlp_reduce(int count, int G, const float* __restrict__ gpart, const float* __restrict__ lpart,
+                             float inv_b, float* __restrict__ g, float* __restrict__ loss, bool adam,
+                             float* __restrict__ p, float* __restrict__ m, float* __restrict__ v, float lr,
+                             float beta1, float beta2, float eps, float bc1, float bc2) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) {
+        float l = 0.0f;
+        for (int w = 0; w < G; ++w) l += lpart[w];
+        *loss = l * inv_b;
+    }
     if (i >= count) return;
-    const float gi = g[i];
-    const float mi = beta1 * m[i] + (1.0f - beta1) * gi;
-    const float vi = beta2 * v[i] + (1.0f - beta2) * gi * gi;
-    m[i] = mi;
-    v[i] = vi;
-    p[i] -= lr * (mi / bc1) / (sqrtf(vi / bc2) + eps);
+    float gi = 0.0f;
+    for (int w = 0; w < G; ++w) gi += gpart[(size_t)w * count + i];
+    g[i] = gi;
+    if (adam) {
+        const float mi = beta1 * m[i] + (1.0f - beta1) * gi;
+        const float vi = beta2 * v[i] + (1.0f - beta2) * gi * gi;
+        m[i] = mi;
+        v[i] = vi;
+        p[i] -= lr * (mi / bc1) / (sqrtf(vi / bc2) + eps);
+    }
 }
 
 thread_local char g_mlp_err[256] = "";
@@ -201,6 +221,8 @@ struct spires_mlp {
     float* m = nullptr;      // Adam first moment
     float* v = nullptr;      // Adam second moment
     float* loss = nullptr;   // [1]
+    float* gpart = nullptr;  // [kMaxGroups x count] partial gradients
+    float* lpart = nullptr;  // [kMaxGroups] partial squared errors
     float* X = nullptr;      // staging [cap x in]
     float* T = nullptr;      // staging [cap]
     float* y = nullptr;      // staging [cap]
@@ -217,6 +239,25 @@ struct spires_mlp {
             return SPIRES_ERR_INTERNAL;                             \
         }                                                           \
     } while (0)
+
+static int mlp_groups(int B) { return std::max(1, std::min(kMaxGroups, (B + kMlpSlice - 1) / kMlpSlice)); }
+
+// Gradient of the staged batch into n->g and n->loss; with adam, one Adam step.
+static void mlp_grad_launch(spires_mlp* n, int B, bool adam, float lr) {
+    const int G = mlp_groups(B);
+    hipLaunchKernelGGL(k_mlp_grad_part, dim3(G), dim3(kMlpThreads), 0, n->stream, n->sh, n->p, n->X, n->T, B,
+                       n->gpart, n->lpart);
+    const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+    float bc1 = 1.0f, bc2 = 1.0f;
+    if (adam) {
+        ++n->step;
+        bc1 = 1.0f - std::pow(b1, (float)n->step);
+        bc2 = 1.0f - std::pow(b2, (float)n->step);
+    }
+    hipLaunchKernelGGL(k_mlp_reduce, dim3((n->sh.count + 255) / 256), dim3(256), 0, n->stream, n->sh.count, G,
+                       n->gpart, n->lpart, 1.0f / (float)B, n->g, n->loss, adam, n->p, n->m, n->v, lr, b1, b2, eps,
+                       bc1, bc2);
+}
 
 static spires_status mlp_stage(spires_mlp* n, int B) {
     if (B <= n->cap) return SPIRES_OK;
@@ -254,6 +295,8 @@ spires_status spires_ext_mlp_create(int in_dim, int hidden, unsigned long long s
     if ((e = hipStreamCreate(&n->stream)) != hipSuccess || (e = hipMalloc(&n->p, bytes)) != hipSuccess ||
         (e = hipMalloc(&n->g, bytes)) != hipSuccess || (e = hipMalloc(&n->m, bytes)) != hipSuccess ||
         (e = hipMalloc(&n->v, bytes)) != hipSuccess || (e = hipMalloc(&n->loss, sizeof(float))) != hipSuccess ||
+        (e = hipMalloc(&n->gpart, bytes * kMaxGroups)) != hipSuccess ||
+        (e = hipMalloc(&n->lpart, sizeof(float) * kMaxGroups)) != hipSuccess ||
         (e = hipMemcpy(n->p, h.data(), bytes, hipMemcpyHostToDevice)) != hipSuccess ||
         (e = hipMemset(n->m, 0, bytes)) != hipSuccess || (e = hipMemset(n->v, 0, bytes)) != hipSuccess) {
         mlp_set_error(hipGetErrorString(e));
@@ -267,6 +310,7 @@ spires_status spires_ext_mlp_create(int in_dim, int hidden, unsigned long long s
 void spires_ext_mlp_destroy(spires_mlp* n) {
     if (!n) return;
     (void)hipFree(n->p); (void)hipFree(n->g); (void)hipFree(n->m); (void)hipFree(n->v); (void)hipFree(n->loss);
+    (void)hipFree(n->gpart); (void)hipFree(n->lpart);
     (void)hipFree(n->X); (void)hipFree(n->T); (void)hipFree(n->y);
     if (n->stream) (void)hipStreamDestroy(n->stream);
     delete n;
@@ -297,7 +341,8 @@ spires_status spires_ext_mlp_predict(spires_mlp* n, const float* X, int B, float
     spires_status rc = mlp_stage(n, B);
     if (rc) return rc;
     MLP_CK(hipMemcpyAsync(n->X, X, (size_t)B * n->sh.in * sizeof(float), hipMemcpyHostToDevice, n->stream));
-    hipLaunchKernelGGL(k_mlp_predict, dim3(1), dim3(kMlpThreads), 0, n->stream, n->sh, n->p, n->X, B, n->y);
+    hipLaunchKernelGGL(k_mlp_predict, dim3(mlp_groups(B)), dim3(kMlpThreads), 0, n->stream, n->sh, n->p, n->X, B,
+                       n->y);
     MLP_CK(hipGetLastError());
     MLP_CK(hipMemcpyAsync(y, n->y, (size_t)B * sizeof(float), hipMemcpyDeviceToHost, n->stream));
     MLP_CK(hipStreamSynchronize(n->stream));
@@ -311,8 +356,7 @@ spires_status spires_ext_mlp_gradient(spires_mlp* n, const float* X, const float
     if (rc) return rc;
     MLP_CK(hipMemcpyAsync(n->X, X, (size_t)B * n->sh.in * sizeof(float), hipMemcpyHostToDevice, n->stream));
     MLP_CK(hipMemcpyAsync(n->T, T, (size_t)B * sizeof(float), hipMemcpyHostToDevice, n->stream));
-    hipLaunchKernelGGL(k_mlp_grad, dim3(1), dim3(kMlpThreads), 0, n->stream, n->sh, n->p, n->X, n->T, B, n->g,
-                       n->loss);
+    mlp_grad_launch(n, B, false, 0.0f);
     MLP_CK(hipGetLastError());
     if (grad)
         MLP_CK(hipMemcpyAsync(grad, n->g, (size_t)n->sh.count * sizeof(float), hipMemcpyDeviceToHost, n->stream));
@@ -328,15 +372,7 @@ spires_status spires_ext_mlp_train(spires_mlp* n, const float* X, const float* T
     if (rc) return rc;
     MLP_CK(hipMemcpyAsync(n->X, X, (size_t)B * n->sh.in * sizeof(float), hipMemcpyHostToDevice, n->stream));
     MLP_CK(hipMemcpyAsync(n->T, T, (size_t)B * sizeof(float), hipMemcpyHostToDevice, n->stream));
-    const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
-    for (int e = 0; e < epochs; ++e) {
-        hipLaunchKernelGGL(k_mlp_grad, dim3(1), dim3(kMlpThreads), 0, n->stream, n->sh, n->p, n->X, n->T, B, n->g,
-                           n->loss);
-        ++n->step;
-        const float bc1 = 1.0f - std::pow(b1, (float)n->step), bc2 = 1.0f - std::pow(b2, (float)n->step);
-        hipLaunchKernelGGL(k_mlp_adam, dim3((n->sh.count + 255) / 256), dim3(256), 0, n->stream, n->p, n->g, n->m,
-                           n->v, n->sh.count, lr, b1, b2, eps, bc1, bc2);
-    }
+    for (int e = 0; e < epochs; ++e) mlp_grad_launch(n, B, true, lr);
     MLP_CK(hipGetLastError());
     if (loss) MLP_CK(hipMemcpyAsync(loss, n->loss, sizeof(float), hipMemcpyDeviceToHost, n->stream));
     MLP_CK(hipStreamSynchronize(n->stream));

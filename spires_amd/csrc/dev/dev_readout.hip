@@ -17,34 +17,71 @@
 #include <rocsolver/rocsolver.h>
 
 #include <cmath>
+#include <mutex>
 #include <vector>
 
 namespace sdev {
 
-struct ReadoutWS {
+// One rocBLAS handle and one grow-only device workspace per device, shared by
+// every reservoir (creating a handle and allocating the n x n systems per fit
+// would cost more than the fit itself for the sizes a search evaluates).
+// Users hold the device's lock for the whole call; the handle is pointed at
+// the caller's stream each time.
+struct ReadoutCtx {
+    std::mutex mu;
     rocblas_handle h = nullptr;
+    double* ws = nullptr; size_t ws_cap = 0;           // doubles
+    rocblas_int* iw = nullptr; size_t iw_cap = 0;
 };
 
-static rocblas_handle get_handle(spires_dev* g) {
-    ReadoutWS* ws = (ReadoutWS*)g->readout_ws;
-    if (!ws) {
-        ws = new ReadoutWS();
-        if (rocblas_create_handle(&ws->h) != rocblas_status_success) { delete ws; return nullptr; }
-        g->readout_ws = ws;
-    }
-    rocblas_set_stream(ws->h, g->stream);
-    rocblas_set_pointer_mode(ws->h, rocblas_pointer_mode_host);
-    return ws->h;
+static ReadoutCtx* readout_ctx() {
+    constexpr int kMaxDevices = 64;
+    static ReadoutCtx ctx[kMaxDevices];
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) return nullptr;
+    return &ctx[dev];
 }
 
-void readout_release(spires_dev* g) {
-    ReadoutWS* ws = (ReadoutWS*)g->readout_ws;
-    if (ws) {
-        if (ws->h) rocblas_destroy_handle(ws->h);
-        delete ws;
+// Locks the current device's context, creates its handle on first use and
+// binds it to g's stream.
+struct ReadoutLock {
+    ReadoutCtx* c = nullptr;
+    std::unique_lock<std::mutex> lk;
+    rocblas_handle h = nullptr;
+    explicit ReadoutLock(spires_dev* g) : c(readout_ctx()) {
+        if (!c) return;
+        lk = std::unique_lock<std::mutex>(c->mu);
+        if (!c->h && rocblas_create_handle(&c->h) != rocblas_status_success) c->h = nullptr;
+        if (!c->h) return;
+        rocblas_set_stream(c->h, g->stream);
+        rocblas_set_pointer_mode(c->h, rocblas_pointer_mode_host);
+        h = c->h;
     }
-    g->readout_ws = nullptr;
-}
+    // Device scratch of at least `doubles` / `ints` entries (contents undefined).
+    int reserve(spires_dev* g, size_t doubles, size_t ints) {
+        if (doubles > c->ws_cap) {
+            (void)hipStreamSynchronize(g->stream);
+            dfree(c->ws);
+            c->ws = nullptr;
+            c->ws_cap = 0;
+            int rc = dalloc(&c->ws, doubles);
+            if (rc) return rc;
+            c->ws_cap = doubles;
+        }
+        if (ints > c->iw_cap) {
+            (void)hipStreamSynchronize(g->stream);
+            dfree(c->iw);
+            c->iw = nullptr;
+            c->iw_cap = 0;
+            int rc = dalloc(&c->iw, ints);
+            if (rc) return rc;
+            c->iw_cap = ints;
+        }
+        return 0;
+    }
+};
+
+void readout_release(spires_dev*) {}
 
 #define SDEV_RB(x)                                                                  \
     do {                                                                            \
@@ -180,7 +217,8 @@ static inline unsigned nb(size_t n, unsigned b) { return (unsigned)((n + b - 1) 
 // Y [T x Dout] = X [T x N] W_out' for a recorded state matrix (row-major).
 int outputs_from_states(spires_dev* g, const double* states, size_t T, double* out) {
     if (g->dout == 0 || T == 0) return 0;
-    rocblas_handle h = get_handle(g);
+    ReadoutLock lock(g);
+    rocblas_handle h = lock.h;
     if (!h) { set_error("rocblas_create_handle failed"); return SPIRES_DEV_EINTERNAL; }
     const double one = 1.0, zero = 0.0;
     // column-major view: Y' (Dout x T) = W_out (Dout x N, ld N as row-major -> use transpose) * X' (N x T)
@@ -210,19 +248,28 @@ int spires_dev_ridge_rows(spires_dev* g, size_t row0, size_t row1, const double*
     const int n = g->n, D = g->dout;
     const int T = (int)(row1 - row0);
     if (D == 0) return 0;
-    rocblas_handle h = get_handle(g);
+    ReadoutLock lock(g);
+    rocblas_handle h = lock.h;
     if (!h) { set_error("rocblas_create_handle failed"); return SPIRES_DEV_EINTERNAL; }
-    double *A = nullptr, *A0 = nullptr, *B = nullptr, *X = nullptr, *R = nullptr, *Yd = nullptr;
-    rocblas_int* info = nullptr;
-    rocblas_int* ipiv = nullptr;
-    int rc = 0;
-    auto cleanup = [&]() { dfree(A); dfree(A0); dfree(B); dfree(X); dfree(R); dfree(Yd); dfree(info); dfree(ipiv); };
     const size_t nn = (size_t)n * n, nd = (size_t)n * D;
-    if ((rc = dalloc(&A, nn)) || (rc = dalloc(&A0, nn)) || (rc = dalloc(&B, nd)) || (rc = dalloc(&X, nd)) ||
-        (rc = dalloc(&R, nd)) || (rc = dalloc(&Yd, (size_t)T * D)) || (rc = dalloc(&info, 1))) {
-        cleanup();
-        return rc;
-    }
+    // workspace: A, A0 [n x n]; B, X, R [n x D]; Y [T x D]; fast mode also
+    // hi, lo (bf16 [n x T]) and G1, C (f32 [n x n]); info + pivots
+    const size_t cnt = (size_t)n * T;
+    auto al = [](size_t bytes) { return (bytes + 255) / 256 * 256; };   // 256-byte aligned pieces
+    const size_t fast_extra = g->mode == SPIRES_DEV_MODE_FAST ? (2 * al(nn * 4) + 2 * al(cnt * 2)) / 8 + 32 : 0;
+    int rc = lock.reserve(g, 2 * nn + 3 * nd + (size_t)T * D + fast_extra, (size_t)n + 1);
+    if (rc) return rc;
+    double* A = lock.c->ws;
+    double* A0 = A + nn;
+    double* B = A0 + nn;
+    double* X = B + nd;
+    double* R = X + nd;
+    double* Yd = R + nd;
+    char* fast_ws = reinterpret_cast<char*>(Yd + (size_t)T * D);
+    fast_ws += (256 - reinterpret_cast<uintptr_t>(fast_ws) % 256) % 256;
+    rocblas_int* info = lock.c->iw;
+    rocblas_int* ipiv = info + 1;
+    auto cleanup = [&]() {};
     const double* Xs = g->rec_state + washout * (size_t)n;   // column-major n x T (ld n) == X'
     const double one = 1.0, zero = 0.0, mone = -1.0;
     H(hipMemcpyAsync(Yd, Y, (size_t)T * D * 8, hipMemcpyHostToDevice, g->stream));
@@ -232,15 +279,10 @@ int spires_dev_ridge_rows(spires_dev* g, size_t row0, size_t row1, const double*
     rocblas_status s1 = rocblas_status_success;
     if (g->mode == SPIRES_DEV_MODE_FAST) {
         // Gram from split-BF16 products with FP32 accumulation: X'X ~ hi'hi + hi'lo + lo'hi
-        unsigned short *hi = nullptr, *lo = nullptr;
-        float *G1 = nullptr, *Cx = nullptr;
-        const size_t cnt = (size_t)n * T;
-        int rc2 = 0;
-        if ((rc2 = dalloc(&hi, cnt)) || (rc2 = dalloc(&lo, cnt)) || (rc2 = dalloc(&G1, nn)) || (rc2 = dalloc(&Cx, nn))) {
-            dfree(hi); dfree(lo); dfree(G1); dfree(Cx);
-            cleanup();
-            return rc2;
-        }
+        float* G1 = reinterpret_cast<float*>(fast_ws);
+        float* Cx = reinterpret_cast<float*>(fast_ws + al(nn * 4));
+        unsigned short* hi = reinterpret_cast<unsigned short*>(fast_ws + 2 * al(nn * 4));
+        unsigned short* lo = reinterpret_cast<unsigned short*>(fast_ws + 2 * al(nn * 4) + al(cnt * 2));
         hipLaunchKernelGGL(k_split_bf16, dim3(nb(cnt, 256)), dim3(256), 0, g->stream, cnt, Xs, hi, lo);
         const float onef = 1.0f, zerof = 0.0f;
         s1 = rocblas_gemm_ex(h, rocblas_operation_none, rocblas_operation_transpose, n, n, T, &onef, hi,
@@ -253,8 +295,6 @@ int spires_dev_ridge_rows(spires_dev* g, size_t row0, size_t row1, const double*
                                  rocblas_datatype_f32_r, n, Cx, rocblas_datatype_f32_r, n, rocblas_datatype_f32_r,
                                  rocblas_gemm_algo_standard, 0, 0);
         hipLaunchKernelGGL(k_combine_gram, dim3(nb(nn, 256)), dim3(256), 0, g->stream, n, G1, Cx, lambda, A);
-        H(hipStreamSynchronize(g->stream));
-        dfree(hi); dfree(lo); dfree(G1); dfree(Cx);
     } else {
         // A = X'X (lower) in FP64
         s1 = rocblas_dsyrk(h, rocblas_fill_lower, rocblas_operation_none, n, T, &one, Xs, n, &zero, A, n);
@@ -273,7 +313,6 @@ int spires_dev_ridge_rows(spires_dev* g, size_t row0, size_t row1, const double*
     H(hipStreamSynchronize(g->stream));
     bool use_lu = (s3 != rocblas_status_success) || hinfo != 0;
     if (use_lu) {
-        if ((rc = dalloc(&ipiv, (size_t)n))) { cleanup(); return rc; }
         H(hipMemcpyAsync(A, A0, nn * 8, hipMemcpyDeviceToDevice, g->stream));
         rocblas_status s4 = rocsolver_dgetrf(h, n, n, A, n, ipiv, info);
         rocblas_int i2 = 0;
@@ -320,6 +359,154 @@ int spires_dev_ridge_rows(spires_dev* g, size_t row0, size_t row1, const double*
     cleanup();
     if (herr != hipSuccess) return hip_fail(herr, "ridge", __LINE__);
     return 0;
+}
+
+int spires_dev_ridge_rows_batch(spires_dev* const* gs, int count, size_t row0, size_t row1, const double* const* Y,
+                                const double* lambda, double* info_out) {
+    if (!gs || count < 1 || !Y || !lambda) return SPIRES_DEV_EINVAL;
+    spires_dev* g0 = gs[0];
+    if (!g0) return SPIRES_DEV_EINVAL;
+    const int n = g0->n, D = g0->dout;
+    bool same = true;
+    for (int r = 0; r < count; ++r) {
+        if (!gs[r] || !Y[r]) return SPIRES_DEV_EINVAL;
+        if (!gs[r]->rec_state || row0 >= row1 || row1 > gs[r]->rec_rows) { set_error("no recorded states for ridge"); return SPIRES_DEV_EINVAL; }
+        same = same && gs[r]->n == n && gs[r]->dout == D && gs[r]->mode == SPIRES_DEV_MODE_EXACT;
+    }
+    if (!same || count == 1) {   // one system at a time
+        for (int r = 0; r < count; ++r) {
+            int rc = spires_dev_ridge_rows(gs[r], row0, row1, Y[r], lambda[r], info_out ? info_out + 2 * r : nullptr);
+            if (rc) return rc;
+        }
+        return 0;
+    }
+    if (D == 0) return 0;
+    hipError_t herr = hipSuccess;
+    auto H = [&](hipError_t e) { if (e != hipSuccess && herr == hipSuccess) herr = e; };
+    for (int r = 0; r < count; ++r) H(hipStreamSynchronize(gs[r]->stream));   // records complete
+    if (herr != hipSuccess) return hip_fail(herr, "ridge batch", __LINE__);
+    ReadoutLock lock(g0);
+    rocblas_handle h = lock.h;
+    if (!h) { set_error("rocblas_create_handle failed"); return SPIRES_DEV_EINTERNAL; }
+    const int T = (int)(row1 - row0);
+    const size_t nn = (size_t)n * n, nd = (size_t)n * D, td = (size_t)T * D;
+    // workspace: A, A0 [count][n x n]; B, X, R [count][n x D]; Y [count][T x D];
+    // info [count] + pivots [n]
+    int rc = lock.reserve(g0, (size_t)count * (2 * nn + 3 * nd + td), (size_t)count + n);
+    if (rc) return rc;
+    double* A = lock.c->ws;
+    double* A0 = A + (size_t)count * nn;
+    double* B = A0 + (size_t)count * nn;
+    double* X = B + (size_t)count * nd;
+    double* R = X + (size_t)count * nd;
+    double* Yd = R + (size_t)count * nd;
+    rocblas_int* info = lock.c->iw;
+    rocblas_int* ipiv = info + count;
+    hipStream_t st = g0->stream;
+    const double one = 1.0, zero = 0.0, mone = -1.0;
+    rocblas_status sb = rocblas_status_success;
+    auto B_ok = [&](rocblas_status x) { if (x != rocblas_status_success && sb == rocblas_status_success) sb = x; };
+    for (int r = 0; r < count; ++r) {
+        const double* Xs = gs[r]->rec_state + row0 * (size_t)n;   // column-major n x T == X'
+        H(hipMemcpyAsync(Yd + r * td, Y[r], td * 8, hipMemcpyHostToDevice, st));
+        B_ok(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_transpose, n, D, T, &one, Xs, n, Yd + r * td, D,
+                           &zero, B + r * nd, n));
+        B_ok(rocblas_dsyrk(h, rocblas_fill_lower, rocblas_operation_none, n, T, &one, Xs, n, &zero, A + r * nn, n));
+        hipLaunchKernelGGL(k_ridge_diag_sym, dim3(nb(nn, 256)), dim3(256), 0, st, n, A + r * nn, lambda[r]);
+    }
+    H(hipMemcpyAsync(A0, A, (size_t)count * nn * 8, hipMemcpyDeviceToDevice, st));
+    H(hipMemcpyAsync(X, B, (size_t)count * nd * 8, hipMemcpyDeviceToDevice, st));
+    B_ok(rocsolver_dpotrf_strided_batched(h, rocblas_fill_lower, n, A, n, (rocblas_stride)nn, info, count));
+    std::vector<rocblas_int> hinfo(count, 0);
+    H(hipMemcpyAsync(hinfo.data(), info, count * sizeof(rocblas_int), hipMemcpyDeviceToHost, st));
+    H(hipStreamSynchronize(st));
+    if (sb != rocblas_status_success || herr != hipSuccess) {
+        set_error("batched ridge factorisation failed");
+        return SPIRES_DEV_EINTERNAL;
+    }
+    B_ok(rocsolver_dpotrs_strided_batched(h, rocblas_fill_lower, n, D, A, n, (rocblas_stride)nn, X, n,
+                                          (rocblas_stride)nd, count));
+    // systems that are not positive definite: LU with partial pivoting, one at a time
+    std::vector<char> lu(count, 0), singular(count, 0);
+    for (int r = 0; r < count; ++r) {
+        if (hinfo[r] == 0) continue;
+        lu[r] = 1;
+        H(hipMemcpyAsync(A + r * nn, A0 + r * nn, nn * 8, hipMemcpyDeviceToDevice, st));
+        B_ok(rocsolver_dgetrf(h, n, n, A + r * nn, n, ipiv, info));
+        rocblas_int i2 = 0;
+        H(hipMemcpyAsync(&i2, info, sizeof(i2), hipMemcpyDeviceToHost, st));
+        H(hipStreamSynchronize(st));
+        if (i2 != 0) { singular[r] = 1; continue; }
+        H(hipMemcpyAsync(X + r * nd, B + r * nd, nd * 8, hipMemcpyDeviceToDevice, st));
+        B_ok(rocsolver_dgetrs(h, rocblas_operation_none, n, D, A + r * nn, n, ipiv, X + r * nd, n));
+        // keep this system's pivots next to its factor for the refinement solves
+        for (int it = 0; it < 2; ++it) {
+            H(hipMemcpyAsync(R + r * nd, B + r * nd, nd * 8, hipMemcpyDeviceToDevice, st));
+            B_ok(rocblas_dsymm(h, rocblas_side_left, rocblas_fill_lower, n, D, &mone, A0 + r * nn, n, X + r * nd, n, &one,
+                               R + r * nd, n));
+            B_ok(rocsolver_dgetrs(h, rocblas_operation_none, n, D, A + r * nn, n, ipiv, R + r * nd, n));
+            hipLaunchKernelGGL(k_axpy, dim3(nb(nd, 256)), dim3(256), 0, st, nd, R + r * nd, X + r * nd);
+        }
+    }
+    // iterative refinement of the Cholesky solutions: R = B - A0 X ; solve ; X += R
+    for (int it = 0; it < 2; ++it) {
+        H(hipMemcpyAsync(R, B, (size_t)count * nd * 8, hipMemcpyDeviceToDevice, st));
+        B_ok(rocblas_dsymm_strided_batched(h, rocblas_side_left, rocblas_fill_lower, n, D, &mone, A0, n,
+                                           (rocblas_stride)nn, X, n, (rocblas_stride)nd, &one, R, n,
+                                           (rocblas_stride)nd, count));
+        B_ok(rocsolver_dpotrs_strided_batched(h, rocblas_fill_lower, n, D, A, n, (rocblas_stride)nn, R, n,
+                                              (rocblas_stride)nd, count));
+        for (int r = 0; r < count; ++r)
+            if (!lu[r]) hipLaunchKernelGGL(k_axpy, dim3(nb(nd, 256)), dim3(256), 0, st, nd, R + r * nd, X + r * nd);
+    }
+    if (info_out) {
+        H(hipMemcpyAsync(R, B, (size_t)count * nd * 8, hipMemcpyDeviceToDevice, st));
+        B_ok(rocblas_dsymm_strided_batched(h, rocblas_side_left, rocblas_fill_lower, n, D, &mone, A0, n,
+                                           (rocblas_stride)nn, X, n, (rocblas_stride)nd, &one, R, n,
+                                           (rocblas_stride)nd, count));
+        std::vector<double> hr((size_t)count * nd), hb((size_t)count * nd);
+        H(hipMemcpyAsync(hr.data(), R, hr.size() * 8, hipMemcpyDeviceToHost, st));
+        H(hipMemcpyAsync(hb.data(), B, hb.size() * 8, hipMemcpyDeviceToHost, st));
+        H(hipStreamSynchronize(st));
+        for (int r = 0; r < count; ++r) {
+            double rel = 0.0;
+            for (int o = 0; o < D; ++o) {
+                double a = 0, b = 0;
+                for (int i = 0; i < n; ++i) {
+                    const double x = hr[r * nd + (size_t)o * n + i], y = hb[r * nd + (size_t)o * n + i];
+                    a += x * x;
+                    b += y * y;
+                }
+                if (b > 0) rel = fmax(rel, sqrt(a / b));
+            }
+            info_out[2 * r] = singular[r] ? -1.0 : (lu[r] ? 1.0 : 0.0);
+            info_out[2 * r + 1] = rel;
+        }
+    }
+    for (int r = 0; r < count; ++r)   // singular systems keep their W_out, as the reference does
+        if (!singular[r]) H(hipMemcpyAsync(gs[r]->W_out, X + r * nd, nd * 8, hipMemcpyDeviceToDevice, st));
+    H(hipStreamSynchronize(st));
+    if (sb != rocblas_status_success) { set_error("batched ridge solve failed"); return SPIRES_DEV_EINTERNAL; }
+    if (herr != hipSuccess) return hip_fail(herr, "ridge batch", __LINE__);
+    return 0;
+}
+
+int spires_dev_predict_records(spires_dev* g, size_t row0, size_t rows, double* out) {
+    if (!g || (!out && rows)) return SPIRES_DEV_EINVAL;
+    if (!g->rec_state || row0 + rows > g->rec_rows) { set_error("rows beyond the recorded states"); return SPIRES_DEV_EINVAL; }
+    if (rows == 0 || g->dout == 0) return 0;
+    ReadoutLock lock(g);
+    rocblas_handle h = lock.h;
+    if (!h) { set_error("rocblas_create_handle failed"); return SPIRES_DEV_EINTERNAL; }
+    const size_t cnt = rows * (size_t)g->dout;
+    int rc = lock.reserve(g, cnt, 0);
+    if (rc) return rc;
+    const double one = 1.0, zero = 0.0;
+    SDEV_RB(rocblas_dgemm(h, rocblas_operation_transpose, rocblas_operation_none, g->dout, (int)rows, g->n, &one,
+                          g->W_out, g->n, g->rec_state + row0 * (size_t)g->n, g->n, &zero, lock.c->ws, g->dout));
+    hipError_t e = hipMemcpyAsync(out, lock.c->ws, cnt * 8, hipMemcpyDeviceToHost, g->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(g->stream);
+    return e == hipSuccess ? 0 : hip_fail(e, "predict records", __LINE__);
 }
 
 int spires_dev_rls(spires_dev* g, const double* Y, size_t T, double delta, double lambda) {

@@ -627,6 +627,22 @@ spires_status spires_ext_fit_ridge_records(spires_reservoir* r, size_t row0, siz
     return dev_status(spires_dev_ridge_rows(r->dev, row0, row1, Y, lambda, info));
 }
 
+spires_status spires_ext_predict_records(spires_reservoir* r, size_t row0, size_t rows, double* out) {
+    if (!r || !r->dev || (!out && rows)) return SPIRES_ERR_INVALID_ARG;
+    return dev_status(spires_dev_predict_records(r->dev, row0, rows, out));
+}
+
+spires_status spires_ext_fit_ridge_records_batch(spires_reservoir* const* rs, int count, size_t row0, size_t row1,
+                                                 const double* const* Y, const double* lambda, double* info) {
+    if (!rs || count < 1 || !Y || !lambda) return SPIRES_ERR_INVALID_ARG;
+    std::vector<spires_dev*> devs(count);
+    for (int i = 0; i < count; ++i) {
+        if (!rs[i] || !rs[i]->dev || !Y[i]) return SPIRES_ERR_INVALID_ARG;
+        devs[i] = rs[i]->dev;
+    }
+    return dev_status(spires_dev_ridge_rows_batch(devs.data(), count, row0, row1, Y, lambda, info));
+}
+
 spires_status spires_ext_run_resident_batch(spires_reservoir* const* rs, int count, size_t t0, size_t T,
                                             unsigned flags) {
     if (!rs || count < 1) return SPIRES_ERR_INVALID_ARG;

@@ -52,6 +52,13 @@ def config1(n=4096, T=20000, T_mem=512, seed=1):
     return cfg, csr, W_in, u
 
 
+def algorithmic_bytes_per_neuron_step(n, nnz, L):
+    """Bytes one neuron-step must move in the FP64 exact step: the L-1 history
+    samples, the new ring sample, the CSR row (int32 col + f64 value per edge,
+    int64 row pointer), W_in, the spike byte and the state-record write."""
+    return 8 * (L - 1) + 8 + (nnz / n) * 12 + 8 + 8 + 1 + 8
+
+
 def delayed_recall_targets(u, delays):
     """y[t, j] = u[t - d_j] (zero before the series starts)."""
     T = u.shape[0]

@@ -37,3 +37,16 @@ def test_spearman_known_values():
     a = np.arange(10.0)
     assert ag.spearman(a, a ** 3) == 1.0
     assert ag.spearman(a, -a) == -1.0
+
+
+def test_latin_hypercube_strata_and_units():
+    n = 64
+    th = ag.latin_hypercube(n, seed=3)
+    z = ag.to_unit(th)
+    # every axis has exactly one sample per stratum [k/n, (k+1)/n) (T_mem is floored, so allow its edge)
+    for j in (0, 1, 2, 3, 5):
+        assert np.array_equal(np.sort(np.floor(z[:, j] * n).astype(int)), np.arange(n))
+    assert np.all(th[:, 4] == np.floor(th[:, 4])) and th[:, 4].min() >= 64 and th[:, 4].max() <= 1024
+    # to_unit / from_unit round trip on the continuous axes
+    back = ag.to_unit(ag.from_unit(z))
+    assert np.allclose(back[:, [0, 1, 2, 3, 5]], z[:, [0, 1, 2, 3, 5]])
