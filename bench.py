@@ -23,8 +23,13 @@ batched FP64 search over 512 trials at 18000 steps sets the target NRMSE, and
 successive halving and the surrogate-guided search are timed until they come
 within 2 % of it (value = median GPU-seconds over the surrogate-search seeds).
 
+With --fast it runs config 3 (FLIF-GL N=16384, L=1024, BA m=8) in FP64 exact and
+FP32 fast mode on the same inputs and reports both rates, raster agreement, the first
+raster divergence, max |dV| before it and each readout's NRMSE (the block size K is
+picked by timing the candidates on a twin reservoir).
+
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--copy-calibration]
-                       [--agile [--agile-seeds S]]
+                       [--agile [--agile-seeds S]] [--fast [--fast-steps T]]
 """
 import argparse
 import ctypes as C
@@ -202,6 +207,122 @@ def run_agile(args):
     print(json.dumps(line), flush=True)
 
 
+def fast_k_autotune(make, u, Ks=(16, 32, 48, 64), steps=300):
+    """Pick the fast-mode time block K by timing `steps` steps of a twin
+    reservoir per candidate (HIP events on its stream); returns (K, {K: us/step})."""
+    import spires_amd as sa
+    ev = HipEvents()
+    times = {}
+    for K in Ks:
+        r = make()
+        r.set_mode(sa.MODE_FAST, K)
+        r.upload_inputs(u[:steps + 50])
+        r.reserve_records(steps, sa.REC_STATE)
+        r.run_resident(0, 50, sa.REC_STATE)      # warm-up
+        r.sync()
+        e0, e1 = ev.create(), ev.create()
+        ev.record(e0, r.stream())
+        r.run_resident(50, steps, sa.REC_STATE)
+        ev.record(e1, r.stream())
+        r.sync()
+        times[K] = ev.elapsed_ms(e0, e1) * 1e3 / steps
+        r.close()
+    return min(times, key=times.get), times
+
+
+LAMBDAS = (1e-2, 1.0, 1e2)   # ridge lambdas of the config-3 readout comparison
+
+
+def run_fast(args):
+    """Config 3: FLIF-GL N=16384, L=1024, alpha=0.7, Barabasi-Albert m=8, N(0,1)
+    weights at rho=0.9, Mackey-Glass input, T steps, in FP64 exact mode (the
+    reference's arithmetic, bit-exact with the oracle) and in fast mode (FP32
+    MFMA history, split-BF16 Gram) on the same inputs.  Reports both rates,
+    raster agreement, the first raster divergence, max |dV| before it, and the
+    readout NRMSE of each (84-step-ahead forecast, fit on [T/10, 3T/4), for each
+    ridge lambda in LAMBDAS)."""
+    import numpy as np
+    import spires_amd as sa
+    from spires_amd import workloads
+    n, L, T, H = 16384, 1024, args.fast_steps, 84
+    params = workloads.flif_gl_params(alpha=0.7, T_mem=L)
+    cfg, csr, W_in = workloads.gaussian_reservoir(n, "scale_free", 8.0 / (n - 1), 0.9, params, 3, dout=1, gain=1.0)
+    series = workloads.mackey_glass(T + H + 1, seed=0)
+    u = np.ascontiguousarray(series[:T].reshape(T, 1))
+    y = series[H:T + H]
+    make = lambda: sa.Reservoir(cfg, csr=csr, W_in=W_in)
+    K, ktimes = fast_k_autotune(make, u)
+    ev = HipEvents()
+    out = {}
+    res = {}
+    for mode in ("exact", "fast"):
+        r = make()
+        if mode == "fast":
+            r.set_mode(sa.MODE_FAST, K)
+        r.upload_inputs(u)
+        r.reserve_records(T, sa.REC_STATE | sa.REC_RASTER)
+        e0, e1 = ev.create(), ev.create()
+        ev.record(e0, r.stream())
+        r.run_resident(0, T, sa.REC_STATE | sa.REC_RASTER)
+        ev.record(e1, r.stream())
+        r.sync()
+        ms = ev.elapsed_ms(e0, e1)
+        w, sp = T // 10, (3 * T) // 4
+        fits = {}
+        for lam in LAMBDAS:
+            info = r.fit_ridge_records(w, sp, y[w:sp].reshape(-1, 1), lam)
+            pred = r.predict_records(sp, T - sp)[:, 0]
+            gram = "fp64" if mode == "exact" else ("fp64 (split-bf16 Gram not positive definite)"
+                                                     if info[0] & 2 else "split-bf16")
+            fits[f"{lam:g}"] = {"val_nrmse": float(np.sqrt(np.mean((pred - y[sp:T]) ** 2)) / np.std(y[sp:T])),
+                                "gram": gram, "solver": "LU" if info[0] & 1 else "Cholesky", "residual": info[1]}
+        out[mode] = {"us_per_step": ms * 1e3 / T, "neuron_steps_per_s": n * T / (ms * 1e-3), "readout": fits}
+        res[mode] = r
+    # raster agreement and first divergence, in row chunks
+    agree, t_div, chunk = 0, T, 2000
+    for a in range(0, T, chunk):
+        m = min(chunk, T - a)
+        _, Re, _ = res["exact"].read_records(a, m, raster=True)
+        _, Rf, _ = res["fast"].read_records(a, m, raster=True)
+        eq = Re == Rf
+        agree += int(eq.sum())
+        if t_div == T:
+            bad = np.nonzero(~eq.all(axis=1))[0]
+            if len(bad):
+                t_div = a + int(bad[0])
+    dv = 0.0
+    for a in range(0, t_div, chunk):
+        m = min(chunk, t_div - a)
+        Xe, _, _ = res["exact"].read_records(a, m, states=True)
+        Xf, _, _ = res["fast"].read_records(a, m, states=True)
+        dv = max(dv, float(np.abs(Xe - Xf).max()))
+    for r in res.values():
+        r.close()
+    fs = out["fast"]["us_per_step"] * 1e-6
+    tflops = 2.0 * (L - 1) * n / fs / 1e12
+    line = {
+        "metric": "fast mode vs FP64 exact, config 3",
+        "value": out["fast"]["neuron_steps_per_s"],
+        "unit": "neuron-steps/s",
+        "n_gpus": 1, "steps": T, "higher_is_better": True, "dtype": "f32 history / f64 update",
+        "data": "synthetic",
+        "config": {"workload": "FLIF-GL N=16384 L=1024 alpha=0.7, Barabasi-Albert m=8, N(0,1) weights rho=0.9, "
+                               "gain 1.0, Mackey-Glass input, 84-step-ahead readout", "time_steps": T,
+                   "block_k": K},
+        "k_autotune_us_per_step": ktimes,
+        "exact": out["exact"], "fast": out["fast"],
+        "raster_agreement": agree / float(T * n),
+        "first_raster_divergence_step": t_div,
+        "max_abs_dV_before_divergence": dv,
+        "nrmse_fast_minus_exact": {k: out["fast"]["readout"][k]["val_nrmse"] - out["exact"]["readout"][k]["val_nrmse"]
+                                   for k in out["exact"]["readout"]},
+        "speedup_fast_vs_exact": out["exact"]["us_per_step"] / out["fast"]["us_per_step"],
+        "roofline": {"bound": "mfma", "achieved": tflops, "peak": 157.3, "unit": "TFLOP/s", "frac": tflops / 157.3,
+                     "traffic": None, "kernel": "k_fast (GL history 2(L-1) flop per neuron-step on FP32 MFMA)"},
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -213,6 +334,9 @@ def main():
     ap.add_argument("--agile", action="store_true",
                     help="run the config-4 hyperparameter search comparison instead of the step benchmark")
     ap.add_argument("--agile-seeds", type=int, default=6)
+    ap.add_argument("--fast", action="store_true",
+                    help="run the config-3 fast-mode comparison (N=16384, L=1024) instead of the step benchmark")
+    ap.add_argument("--fast-steps", type=int, default=50000)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs one reservoir per device and has no "
@@ -227,6 +351,9 @@ def main():
         sa.build()
     if args.agile:
         run_agile(args)
+        return
+    if args.fast:
+        run_fast(args)
         return
     if args.copy_calibration:
         import torch

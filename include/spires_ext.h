@@ -61,8 +61,9 @@ spires_status spires_ext_read_records(spires_reservoir *r, size_t row0, size_t r
 
 /* Ridge readout with a washout: reset, run u [T x Din], drop the first
  * `washout` states, fit W_out on targets y [(T - washout) x Dout].
- * info (optional, 2 doubles): [0] = 1 if the LU fallback was used,
- * [1] = relative residual of the normal equations after refinement. */
+ * info (optional, 2 doubles): [0] = 1 if the LU fallback was used, plus 2 if
+ * fast mode formed the FP64 Gram because its split-BF16 Gram was not positive
+ * definite; [1] = relative residual of the normal equations after refinement. */
 spires_status spires_ext_train_ridge_washout(spires_reservoir *r, const double *u,
                                              const double *y, size_t T, size_t washout,
                                              double lambda, double *info);

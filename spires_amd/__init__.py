@@ -402,11 +402,12 @@ class Reservoir:
 
     def fit_ridge_records(self, row0, row1, Y, lam):
         """Ridge W_out on recorded state rows [row0, row1) against Y; returns
-        (used_lu, relative residual)."""
+        (flags, relative residual): flags bit 0 = LU fallback, bit 1 = fast
+        mode fell back to the FP64 Gram."""
         Y = _f64(Y).reshape(row1 - row0, -1)
         info = np.zeros(2)
         _check(lib().spires_ext_fit_ridge_records(self.h, row0, row1, _dp(Y), lam, _dp(info)))
-        return bool(info[0]), float(info[1])
+        return int(info[0]), float(info[1])
 
     def predict_records(self, row0, rows):
         """Readout outputs [rows x Dout] of recorded state rows under the current W_out."""

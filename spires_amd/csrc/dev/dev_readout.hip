@@ -3,11 +3,12 @@
 // Ridge (reservoir.c:573-679): the recorded state matrix X [T x N] stays in
 // HBM.  A = X'X (+ lambda I) and B = X'Y are formed on the GPU (in fast mode
 // the Gram comes from split-BF16 products hi'hi + hi'lo + lo'hi accumulated in
-// FP32 on the matrix cores and combined in FP64), then one FP64
-// Cholesky factorisation solves all Dout right-hand sides at once (the
+// FP32 on the matrix cores over chunks of rows and summed in FP64), then one
+// FP64 Cholesky factorisation solves all Dout right-hand sides at once (the
 // reference refactorises A with LU once per output), followed by FP64
-// iterative refinement against the unfactored A.  A matrix that is not
-// positive definite falls back to LU with partial pivoting.
+// iterative refinement against the normal equations (in fast mode their
+// residual comes from X itself in FP64).  A matrix that is not positive
+// definite falls back to LU with partial pivoting.
 //
 // RLS (reservoir.c:681-767) and the delta rule (reservoir.c:244-280) run as
 // fused FP64 kernels on the resident P / W_out.
@@ -16,6 +17,7 @@
 #include <rocblas/rocblas.h>
 #include <rocsolver/rocsolver.h>
 
+#include <algorithm>
 #include <cmath>
 #include <mutex>
 #include <vector>
@@ -119,15 +121,27 @@ __global__ void k_split_bf16(size_t cnt, const double* x, unsigned short* hi, un
     lo[q] = f32_to_bf16((float)(v - (double)bf16_to_f32(h)));
 }
 
-// A = G1 + C + C' (+ lambda on the diagonal), FP32 products combined in FP64.
-__global__ void k_combine_gram(int n, const float* G1, const float* Cx, double lambda, double* A) {
+// A (+)= G1 + C + C' for one chunk of rows: FP32 products combined and summed
+// over chunks in FP64.
+__global__ void k_accum_gram(int n, const float* G1, const float* Cx, double* A, int first) {
     const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= (size_t)n * n) return;
     const int c = (int)(idx / n), r = (int)(idx % n);
-    double v = (double)G1[idx] + (double)Cx[idx] + (double)Cx[(size_t)r * n + c];
-    if (r == c) v += lambda;
-    A[idx] = v;
+    const double v = (double)G1[idx] + (double)Cx[idx] + (double)Cx[(size_t)r * n + c];
+    A[idx] = first ? v : A[idx] + v;
 }
+
+__global__ void k_add_diag(int n, double* A, double lambda) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) A[(size_t)i * n + i] += lambda;
+}
+
+__global__ void k_axpby(size_t n, double a, const double* x, double* y) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) y[i] = y[i] + a * x[i];
+}
+
+constexpr int kGramChunk = 2048;   // rows per split-BF16 product (FP32 accumulation length)
 
 __global__ void k_axpy(size_t n, const double* x, double* y) {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -254,9 +268,11 @@ int spires_dev_ridge_rows(spires_dev* g, size_t row0, size_t row1, const double*
     const size_t nn = (size_t)n * n, nd = (size_t)n * D;
     // workspace: A, A0 [n x n]; B, X, R [n x D]; Y [T x D]; fast mode also
     // hi, lo (bf16 [n x T]) and G1, C (f32 [n x n]); info + pivots
-    const size_t cnt = (size_t)n * T;
+    bool fast = g->mode == SPIRES_DEV_MODE_FAST;
+    const int TC = std::min(T, kGramChunk);
+    const size_t cnt = (size_t)n * TC;
     auto al = [](size_t bytes) { return (bytes + 255) / 256 * 256; };   // 256-byte aligned pieces
-    const size_t fast_extra = g->mode == SPIRES_DEV_MODE_FAST ? (2 * al(nn * 4) + 2 * al(cnt * 2)) / 8 + 32 : 0;
+    const size_t fast_extra = fast ? (2 * al(nn * 4) + 2 * al(cnt * 2) + al((size_t)T * D * 8) + al(nd * 8)) / 8 + 48 : 0;
     int rc = lock.reserve(g, 2 * nn + 3 * nd + (size_t)T * D + fast_extra, (size_t)n + 1);
     if (rc) return rc;
     double* A = lock.c->ws;
@@ -277,24 +293,36 @@ int spires_dev_ridge_rows(spires_dev* g, size_t row0, size_t row1, const double*
     rocblas_status s2 = rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_transpose, n, D, T, &one, Xs, n,
                                       Yd, D, &zero, B, n);
     rocblas_status s1 = rocblas_status_success;
-    if (g->mode == SPIRES_DEV_MODE_FAST) {
-        // Gram from split-BF16 products with FP32 accumulation: X'X ~ hi'hi + hi'lo + lo'hi
+    double* XW = nullptr;   // fast mode: X W [T x D] of the refinement residual
+    double* dX = nullptr;   // fast mode: the last refinement step [n x D]
+    if (fast) {
+        // Gram from split-BF16 products, X'X ~ hi'hi + hi'lo + lo'hi, over chunks of
+        // kGramChunk rows: each chunk accumulates in FP32 on the matrix cores, the
+        // chunks add up in FP64 (the FP32 accumulation length stays kGramChunk).
         float* G1 = reinterpret_cast<float*>(fast_ws);
         float* Cx = reinterpret_cast<float*>(fast_ws + al(nn * 4));
         unsigned short* hi = reinterpret_cast<unsigned short*>(fast_ws + 2 * al(nn * 4));
         unsigned short* lo = reinterpret_cast<unsigned short*>(fast_ws + 2 * al(nn * 4) + al(cnt * 2));
-        hipLaunchKernelGGL(k_split_bf16, dim3(nb(cnt, 256)), dim3(256), 0, g->stream, cnt, Xs, hi, lo);
+        XW = reinterpret_cast<double*>(fast_ws + 2 * al(nn * 4) + 2 * al(cnt * 2));
+        dX = reinterpret_cast<double*>(fast_ws + 2 * al(nn * 4) + 2 * al(cnt * 2) + al((size_t)T * D * 8));
         const float onef = 1.0f, zerof = 0.0f;
-        s1 = rocblas_gemm_ex(h, rocblas_operation_none, rocblas_operation_transpose, n, n, T, &onef, hi,
-                             rocblas_datatype_bf16_r, n, hi, rocblas_datatype_bf16_r, n, &zerof, G1,
-                             rocblas_datatype_f32_r, n, G1, rocblas_datatype_f32_r, n, rocblas_datatype_f32_r,
-                             rocblas_gemm_algo_standard, 0, 0);
-        if (s1 == rocblas_status_success)
-            s1 = rocblas_gemm_ex(h, rocblas_operation_none, rocblas_operation_transpose, n, n, T, &onef, hi,
-                                 rocblas_datatype_bf16_r, n, lo, rocblas_datatype_bf16_r, n, &zerof, Cx,
-                                 rocblas_datatype_f32_r, n, Cx, rocblas_datatype_f32_r, n, rocblas_datatype_f32_r,
+        for (int c0 = 0; c0 < T && s1 == rocblas_status_success; c0 += TC) {
+            const int tc = std::min(TC, T - c0);
+            const size_t cc = (size_t)n * tc;
+            hipLaunchKernelGGL(k_split_bf16, dim3(nb(cc, 256)), dim3(256), 0, g->stream, cc, Xs + (size_t)c0 * n, hi,
+                               lo);
+            s1 = rocblas_gemm_ex(h, rocblas_operation_none, rocblas_operation_transpose, n, n, tc, &onef, hi,
+                                 rocblas_datatype_bf16_r, n, hi, rocblas_datatype_bf16_r, n, &zerof, G1,
+                                 rocblas_datatype_f32_r, n, G1, rocblas_datatype_f32_r, n, rocblas_datatype_f32_r,
                                  rocblas_gemm_algo_standard, 0, 0);
-        hipLaunchKernelGGL(k_combine_gram, dim3(nb(nn, 256)), dim3(256), 0, g->stream, n, G1, Cx, lambda, A);
+            if (s1 == rocblas_status_success)
+                s1 = rocblas_gemm_ex(h, rocblas_operation_none, rocblas_operation_transpose, n, n, tc, &onef, hi,
+                                     rocblas_datatype_bf16_r, n, lo, rocblas_datatype_bf16_r, n, &zerof, Cx,
+                                     rocblas_datatype_f32_r, n, Cx, rocblas_datatype_f32_r, n,
+                                     rocblas_datatype_f32_r, rocblas_gemm_algo_standard, 0, 0);
+            hipLaunchKernelGGL(k_accum_gram, dim3(nb(nn, 256)), dim3(256), 0, g->stream, n, G1, Cx, A, c0 == 0 ? 1 : 0);
+        }
+        hipLaunchKernelGGL(k_add_diag, dim3(nb((size_t)n, 256)), dim3(256), 0, g->stream, n, A, lambda);
     } else {
         // A = X'X (lower) in FP64
         s1 = rocblas_dsyrk(h, rocblas_fill_lower, rocblas_operation_none, n, T, &one, Xs, n, &zero, A, n);
@@ -312,6 +340,21 @@ int spires_dev_ridge_rows(spires_dev* g, size_t row0, size_t row1, const double*
     H(hipMemcpyAsync(&hinfo, info, sizeof(hinfo), hipMemcpyDeviceToHost, g->stream));
     H(hipStreamSynchronize(g->stream));
     bool use_lu = (s3 != rocblas_status_success) || hinfo != 0;
+    bool fp64_gram = !fast;
+    if (use_lu && fast) {
+        // The split-BF16 Gram (about 2^-16 relative) is not positive definite:
+        // lambda is too small for it.  Form the FP64 Gram instead and factor that.
+        fast = false;
+        fp64_gram = true;
+        rocblas_status s5 = rocblas_dsyrk(h, rocblas_fill_lower, rocblas_operation_none, n, T, &one, Xs, n, &zero, A, n);
+        hipLaunchKernelGGL(k_ridge_diag_sym, dim3(nb(nn, 256)), dim3(256), 0, g->stream, n, A, lambda);
+        H(hipMemcpyAsync(A0, A, nn * 8, hipMemcpyDeviceToDevice, g->stream));
+        H(hipMemcpyAsync(X, B, nd * 8, hipMemcpyDeviceToDevice, g->stream));
+        s3 = s5 == rocblas_status_success ? rocsolver_dpotrf(h, rocblas_fill_lower, n, A, n, info) : s5;
+        H(hipMemcpyAsync(&hinfo, info, sizeof(hinfo), hipMemcpyDeviceToHost, g->stream));
+        H(hipStreamSynchronize(g->stream));
+        use_lu = (s3 != rocblas_status_success) || hinfo != 0;
+    }
     if (use_lu) {
         H(hipMemcpyAsync(A, A0, nn * 8, hipMemcpyDeviceToDevice, g->stream));
         rocblas_status s4 = rocsolver_dgetrf(h, n, n, A, n, ipiv, info);
@@ -328,29 +371,68 @@ int spires_dev_ridge_rows(spires_dev* g, size_t row0, size_t row1, const double*
     } else {
         rocsolver_dpotrs(h, rocblas_fill_lower, n, D, A, n, X, n);
     }
-    // iterative refinement: R = B - A0 X ; solve ; X += dX
-    double rel = 0.0;
-    for (int it = 0; it < 2; ++it) {
+    // Residual R = B - A W of the factored system (exact mode: the FP64 Gram;
+    // fast mode: the split-BF16 Gram in FP64), or, with exact_gram in fast
+    // mode, of the FP64 normal equations B - (X'X + lambda I) W formed from X
+    // itself (two products with the state record).
+    auto residual = [&](bool exact_gram) {
         H(hipMemcpyAsync(R, B, nd * 8, hipMemcpyDeviceToDevice, g->stream));
-        rocblas_dsymm(h, rocblas_side_left, rocblas_fill_lower, n, D, &mone, A0, n, X, n, &one, R, n);
+        if (fast && exact_gram) {
+            rocblas_dgemm(h, rocblas_operation_transpose, rocblas_operation_none, T, D, n, &one, Xs, n, X, n, &zero, XW,
+                          T);
+            rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, n, D, T, &mone, Xs, n, XW, T, &one, R, n);
+            hipLaunchKernelGGL(k_axpby, dim3(nb(nd, 256)), dim3(256), 0, g->stream, nd, -lambda, X, R);
+        } else {
+            rocblas_dsymm(h, rocblas_side_left, rocblas_fill_lower, n, D, &mone, A0, n, X, n, &one, R, n);
+        }
+    };
+    auto solve_r = [&]() {   // R <- A^-1 R with the factorisation
         if (use_lu) rocsolver_dgetrs(h, rocblas_operation_none, n, D, A, n, ipiv, R, n);
         else rocsolver_dpotrs(h, rocblas_fill_lower, n, D, A, n, R, n);
-        hipLaunchKernelGGL(k_axpy, dim3(nb(nd, 256)), dim3(256), 0, g->stream, nd, R, X);
-    }
-    if (info_out) {
-        // final relative residual max_o ||B - A0 X|| / ||B||
-        H(hipMemcpyAsync(R, B, nd * 8, hipMemcpyDeviceToDevice, g->stream));
-        rocblas_dsymm(h, rocblas_side_left, rocblas_fill_lower, n, D, &mone, A0, n, X, n, &one, R, n);
-        std::vector<double> hr(nd), hb(nd);
+    };
+    std::vector<double> hr(nd), hb(nd);
+    auto rel_norm = [&]() {  // max over outputs of ||R|| / ||B|| (host)
         H(hipMemcpyAsync(hr.data(), R, nd * 8, hipMemcpyDeviceToHost, g->stream));
         H(hipMemcpyAsync(hb.data(), B, nd * 8, hipMemcpyDeviceToHost, g->stream));
         H(hipStreamSynchronize(g->stream));
+        double rel = 0.0;
         for (int o = 0; o < D; ++o) {
             double a = 0, b = 0;
             for (int i = 0; i < n; ++i) { a += hr[(size_t)o * n + i] * hr[(size_t)o * n + i]; b += hb[(size_t)o * n + i] * hb[(size_t)o * n + i]; }
             if (b > 0) rel = fmax(rel, sqrt(a / b));
         }
-        info_out[0] = use_lu ? 1.0 : 0.0;
+        return rel;
+    };
+    // iterative refinement against the factored system: R = B - A X ; solve ; X += dX
+    for (int it = 0; it < 2; ++it) {
+        residual(false);
+        solve_r();
+        hipLaunchKernelGGL(k_axpy, dim3(nb(nd, 256)), dim3(256), 0, g->stream, nd, R, X);
+    }
+    // fast mode: further steps against the FP64 normal equations, each kept only
+    // while it lowers their residual (the split-BF16 factor is only a
+    // preconditioner there, which may not converge when lambda is small)
+    if (fast) {
+        residual(true);
+        double cur = rel_norm();
+        for (int it = 0; it < 4 && herr == hipSuccess; ++it) {
+            solve_r();
+            hipLaunchKernelGGL(k_axpy, dim3(nb(nd, 256)), dim3(256), 0, g->stream, nd, R, X);
+            H(hipMemcpyAsync(dX, R, nd * 8, hipMemcpyDeviceToDevice, g->stream));   // keep the step
+            residual(true);
+            const double nxt = rel_norm();
+            if (!(nxt < cur)) {   // undo the step
+                hipLaunchKernelGGL(k_axpby, dim3(nb(nd, 256)), dim3(256), 0, g->stream, nd, -1.0, dX, X);
+                break;
+            }
+            cur = nxt;
+        }
+    }
+    double rel = 0.0;
+    if (info_out) {
+        residual(fast);
+        rel = rel_norm();
+        info_out[0] = (use_lu ? 1.0 : 0.0) + (fp64_gram && g->mode == SPIRES_DEV_MODE_FAST ? 2.0 : 0.0);
         info_out[1] = rel;
     }
     // X is column-major n x D == row-major W_out [D x n]

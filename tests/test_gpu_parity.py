@@ -317,6 +317,38 @@ def _nrmse(y, yh):
     return float(np.sqrt(np.mean((y - yh) ** 2)) / np.std(y))
 
 
+def test_fast_readout_split_bf16_gram_against_fp64_solve():
+    """Fast-mode ridge (split-BF16 Gram, FP64 factorisation and refinement) on
+    the fast-mode states against numpy's FP64 ridge on the same states.  With
+    lambda large enough for the split-BF16 Gram (about 2^-16 relative) to stay
+    positive definite, refinement against the FP64 normal equations brings W
+    to the FP64 solution (1e-8 relative); with a tiny lambda the library
+    falls back to the FP64 Gram (flag bit 1) and agrees to solver accuracy."""
+    n, L, T, w = 384, 96, 2400, 200
+    kw, csr, Wi = _gl_gauss(n, L, 41, dout=1)
+    r = sa.Reservoir(sa.make_config(n, **kw), csr=csr, W_in=Wi)
+    r.set_mode(sa.MODE_FAST, 32)
+    u = np.random.default_rng(42).uniform(0, 0.5, (T, 1))
+    y = np.roll(u[:, 0], 3)
+    r.upload_inputs(u)
+    r.reserve_records(T, sa.REC_STATE)
+    r.run_resident(0, T, sa.REC_STATE)
+    X, _, _ = r.read_records(0, T, states=True)
+    Xs = X[w:]
+    for lam in (0.05 * T, 1e-8):
+        flags, resid = r.fit_ridge_records(w, T, y[w:], lam)
+        A = Xs.T @ Xs + lam * np.eye(n)
+        Wn = np.linalg.solve(A, Xs.T @ y[w:])
+        Wg = r.W_out()[0]
+        err = np.abs(Wg - Wn).max() / np.abs(Wn).max()
+        print("lambda", lam, "flags", flags, "residual", resid, "rel err", err, "kappa", np.linalg.cond(A))
+        if lam > 1.0:
+            assert not (flags & 2) and err < 1e-8
+        elif flags & 2:
+            assert err <= max(1e-9, 1e-14 * np.linalg.cond(A))
+    r.close()
+
+
 def test_fast_ridge_split_bf16_matches_fp64_readout():
     """Fast mode (FP32 history, split-BF16 Gram, FP64 Cholesky + refinement)
     against the FP64 exact path on a delayed-recall task: validation NRMSE of
