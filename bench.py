@@ -29,7 +29,7 @@ raster divergence, max |dV| before it and each readout's NRMSE (the block size K
 picked by timing the candidates on a twin reservoir).
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--copy-calibration]
-                       [--agile [--agile-seeds S]] [--fast [--fast-steps T]]
+                       [--agile [--agile-seeds S]] [--fast [--fast-steps T]] [--batched]
 """
 import argparse
 import ctypes as C
@@ -230,6 +230,97 @@ def fast_k_autotune(make, u, Ks=(16, 32, 48, 64), steps=300):
     return min(times, key=times.get), times
 
 
+def run_batched(args):
+    """Config 2: 64 FLIF-GL reservoirs of uneven N (log-U[256, 8192]), L, alpha
+    and topology (ER p=0.05 / BA m=5), N(0,1) weights rescaled to rho, on one
+    13-channel input of AR(1) class sequences, advanced in time blocks of K
+    steps.  Reservoirs that fit the product-wave kernel are packed, per neurons-
+    per-workgroup class, into launches of at most one workgroup per CU; the
+    others run one launch each.  K is picked by timing the first blocks."""
+    import numpy as np
+    import spires_amd as sa
+    from spires_amd import workloads
+    u, lens, labels = workloads.class_sequences()
+    T = u.shape[0]
+    specs = workloads.config2_specs()
+    res, bps = [], []
+    for sp in specs:
+        params = workloads.flif_gl_params(alpha=sp["alpha"], T_mem=sp["T_mem"])
+        cfg, csr, W_in = workloads.gaussian_reservoir(sp["n"], sp["topology"], sp["conn"], sp["rho"], params, sp["seed"],
+                                                      din=13, dout=10, gain=1.0)
+        r = sa.Reservoir(cfg, csr=csr, W_in=W_in)
+        r.upload_inputs(u)
+        r.reserve_records(T, sa.REC_STATE)
+        res.append(r)
+        bps.append(workloads.algorithmic_bytes_per_neuron_step(sp["n"], len(csr[2]), sp["T_mem"]))
+    plans = [r.kernel_plan() for r in res]
+    groups, singles = {}, []
+    for i, pl in enumerate(plans):
+        (groups.setdefault(pl["nb"], []) if pl["kernel"] == 2 else singles).append(i)
+    launches = []
+    for nb, idx in sorted(groups.items()):
+        # launch-mates finish together when their step times are alike: order by
+        # row-pull kind, history length and size
+        idx.sort(key=lambda i: (plans[i]["weights_lds"], plans[i]["row_pull"], specs[i]["T_mem"], -specs[i]["n"]))
+        cur = []
+        for i in idx:
+            if cur and sa.batch_workgroups([res[j] for j in cur + [i]]) == 0:
+                launches.append(cur)
+                cur = []
+            cur.append(i)
+        if cur:
+            launches.append(cur)
+    launches += [[i] for i in singles]
+    total_n = sum(sp["n"] for sp in specs)
+    flags = sa.REC_STATE | sa.REC_APPEND
+
+    def block(t, K):
+        for grp in launches:
+            if len(grp) > 1:
+                sa.run_resident_batch([res[i] for i in grp], t, K, flags)
+            else:
+                res[grp[0]].run_resident(t, K, flags)
+
+    # untimed warm-up until every history is full, then K autotune on the next
+    # blocks of the run (each candidate advances every reservoir)
+    t, ktimes = 1024, {}
+    block(0, t)
+    for K in (64, 256, 1024):
+        t0 = time.perf_counter()
+        block(t, K)
+        ktimes[K] = (time.perf_counter() - t0) / K * 1e6
+        t += K
+    K = min(ktimes, key=ktimes.get)
+    steps = T - t
+    t0 = time.perf_counter()
+    while t < T:
+        k = min(K, T - t)
+        block(t, k)
+        t += k
+    wall = time.perf_counter() - t0
+    value = total_n * steps / wall
+    alg = sum(b * sp["n"] for b, sp in zip(bps, specs)) * steps / wall / 1e9
+    for r in res:
+        r.close()
+    line = {
+        "metric": "batched neuron-steps/s, config 2",
+        "value": value, "unit": "neuron-steps/s", "n_gpus": 1, "steps": steps, "higher_is_better": True,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "64 FLIF-GL reservoirs, N log-U[256,8192] (total %d neurons), ER p=0.05 / BA m=5, "
+                               "alpha U[0.5,1], T_mem in {128,256,512,1024}, rho U[0.7,1.2], N(0,1) weights; "
+                               "13-channel AR(1) class sequences (512 sequences, %d frames)" % (total_n, T),
+                   "block_k": K},
+        "k_autotune_us_per_step": ktimes,
+        "launches_per_block": len(launches),
+        "batched_launches": [[specs[i]["n"] for i in g] for g in launches if len(g) > 1],
+        "single_launches": [dict(n=specs[i]["n"], T_mem=specs[i]["T_mem"], kernel=plans[i]["kernel"]) for i in singles],
+        "ms_per_step": wall / steps * 1e3,
+        "roofline": {"bound": "hbm", "achieved": alg, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": alg / HBM_PEAK_GBS,
+                     "traffic": None, "kernel": "k_persist_tp_batch + k_persist (all launches of a block)"},
+    }
+    print(json.dumps(line), flush=True)
+
+
 LAMBDAS = (1e-2, 1.0, 1e2)   # ridge lambdas of the config-3 readout comparison
 
 
@@ -337,6 +428,8 @@ def main():
     ap.add_argument("--fast", action="store_true",
                     help="run the config-3 fast-mode comparison (N=16384, L=1024) instead of the step benchmark")
     ap.add_argument("--fast-steps", type=int, default=50000)
+    ap.add_argument("--batched", action="store_true",
+                    help="run the config-2 batched workload (64 uneven reservoirs) instead of the step benchmark")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs one reservoir per device and has no "
@@ -354,6 +447,9 @@ def main():
         return
     if args.fast:
         run_fast(args)
+        return
+    if args.batched:
+        run_batched(args)
         return
     if args.copy_calibration:
         import torch

@@ -117,6 +117,31 @@ __device__ __forceinline__ double row_pull(int stype, bool csr_lds, int rs, int 
         }
         return s;
     }
+    if (stype == ST_SIMPLE) {   // CSR slice in HBM: batches of 16 edges, loads issued before the in-order sum
+        const int32_t* cr = col + E0;
+        const double* vr = val + E0;
+        int e = rs;
+        for (; e + 16 <= re; e += 16) {
+            int32_t cj[16];
+            double wv[16];
+#pragma unroll
+            for (int q = 0; q < 16; ++q) { cj[q] = cr[e + q]; wv[q] = vr[e + q]; }
+            unsigned hit = 0;
+#pragma unroll
+            for (int q = 0; q < 16; ++q) {
+                const unsigned pk = encode_col(cj[q], nb, wpg);
+                hit |= ((bits[pk >> 5] >> (pk & 31)) & 1u) << q;
+            }
+#pragma unroll
+            for (int q = 0; q < 16; ++q)
+                if ((hit >> q) & 1u) s = s + wv[q];
+        }
+        for (; e < re; ++e) {
+            const unsigned pk = encode_col(cr[e], nb, wpg);
+            if ((bits[pk >> 5] >> (pk & 31)) & 1u) s = s + vr[e];
+        }
+        return s;
+    }
     for (int e = rs; e < re; ++e) {
         const int64_t k = E0 + e;
         const unsigned pk = csr_lds ? eidx[e] : encode_col(col[k], nb, wpg);

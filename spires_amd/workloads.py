@@ -96,3 +96,37 @@ def mackey_glass(T, seed=0, tau=17.0, beta=0.2, gamma=0.1, n=10, dt=0.1, every=1
         x[k + 1] = xt + dt / 6.0 * (k1 + 2 * k2 + 2 * k3 + k4)
     series = x[lag::sub][washout:washout + T]
     return series.copy()
+
+
+def config2_specs(count=64, seed=2):
+    """Hyperparameters of the batched workload: N log-uniform in [256, 8192],
+    topology alternating Erdos-Renyi p=0.05 / Barabasi-Albert m=5, alpha
+    U[0.5, 1], T_mem in {128, 256, 512, 1024}, rho U[0.7, 1.2]."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(count):
+        n = int(round(float(np.exp(rng.uniform(np.log(256), np.log(8192))))))
+        out.append(dict(n=n, topology="random" if k % 2 == 0 else "scale_free",
+                        conn=0.05 if k % 2 == 0 else 5.0 / (n - 1), alpha=float(rng.uniform(0.5, 1.0)),
+                        T_mem=int(rng.choice([128, 256, 512, 1024])), rho=float(rng.uniform(0.7, 1.2)),
+                        seed=1000 + k))
+    return out
+
+
+def class_sequences(count=512, channels=13, classes=10, phi=0.9, offset_std=0.5, min_len=30, max_len=100, seed=3):
+    """Sequences of AR(1) frames: per channel x_t = phi x_{t-1} + N(0,1), plus a
+    fixed random mean offset (std offset_std) per class.  Returns the frames
+    concatenated [T x channels], the sequence lengths and the labels."""
+    rng = np.random.default_rng(seed)
+    means = rng.normal(0.0, offset_std, (classes, channels))
+    lens = rng.integers(min_len, max_len + 1, count)
+    labels = rng.integers(0, classes, count)
+    frames = []
+    for L, c in zip(lens, labels):
+        x = np.empty((L, channels))
+        prev = np.zeros(channels)
+        for t in range(L):
+            prev = phi * prev + rng.standard_normal(channels)
+            x[t] = prev
+        frames.append(x + means[c])
+    return np.concatenate(frames), lens, labels

@@ -1,7 +1,8 @@
-"""Row-pull variants of the product-wave kernel at N=2048 against the oracle:
-the edge list in LDS (sparse rows), spike-word bitmaps with the weights in LDS
-(dense rows), and bitmaps with the weights read from HBM (dense rows next to a
-long history).  Each case checks the plan picked the variant it names, then
+"""Row-pull variants of the product-wave kernel against the oracle: the edge
+list in LDS (sparse rows), column bitmaps with the weights in LDS (dense
+rows), bitmaps with the weights read from HBM (dense rows next to a long
+history), and the edge list read from HBM in batches (N=4452, where neither
+fits next to the history).  Each case checks the plan picked the variant it names, then
 bit-exact rasters and membrane traces."""
 import math
 
@@ -13,10 +14,11 @@ from oracle import pyoracle as po
 
 pytestmark = pytest.mark.gpu
 
-CASES = [  # connectivity, history length, expected (row_pull, weights_lds, neurons per workgroup)
-    (0.005, 128, (0, 1, 32)),
-    (0.1, 128, (1, 1, 32)),
-    (0.2, 900, (1, 0, 16)),
+CASES = [  # neurons, connectivity, history length, expected (row_pull, weights_lds, neurons per workgroup)
+    (2048, 0.005, 128, (0, 1, 32)),
+    (2048, 0.1, 128, (1, 1, 32)),
+    (2048, 0.2, 900, (1, 0, 16)),
+    (4452, 0.05, 512, (0, 0, 32)),    # neither the edge list nor the bitmaps fit next to the history
 ]
 
 
@@ -31,9 +33,8 @@ def _er(n, conn, seed):
     return rp, cols.astype(np.int32), val, rng.uniform(-1, 1, (n, 1))
 
 
-@pytest.mark.parametrize("conn,L,expect", CASES)
-def test_row_pull_variants_bit_exact(conn, L, expect):
-    n = 2048
+@pytest.mark.parametrize("n,conn,L,expect", CASES)
+def test_row_pull_variants_bit_exact(n, conn, L, expect):
     params = [1.0, 0.0, 0.0, 20.0, 0.7, float(L), 0.05, 2.0]
     rp, ci, val, Wi = _er(n, conn, 21)
     kw = dict(din=1, dout=1, gain=3.0, neuron="flif_gl", params=params)
