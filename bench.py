@@ -234,9 +234,12 @@ def run_batched(args):
     """Config 2: 64 FLIF-GL reservoirs of uneven N (log-U[256, 8192]), L, alpha
     and topology (ER p=0.05 / BA m=5), N(0,1) weights rescaled to rho, on one
     13-channel input of AR(1) class sequences, advanced in time blocks of K
-    steps.  Reservoirs that fit the product-wave kernel are packed, per neurons-
-    per-workgroup class, into launches of at most one workgroup per CU; the
-    others run one launch each.  K is picked by timing the first blocks."""
+    steps.  The reservoirs that fit the product-wave kernel run as one
+    TimeBlock: their launches (packed per neurons-per-workgroup class, at most
+    one workgroup per CU) are captured once in a HIP graph and replayed every
+    block; the others run one launch each per block.  K is picked by timing
+    one block per candidate; the same blocks are also timed with eager
+    launches for comparison."""
     import numpy as np
     import spires_amd as sa
     from spires_amd import workloads
@@ -254,50 +257,73 @@ def run_batched(args):
         res.append(r)
         bps.append(workloads.algorithmic_bytes_per_neuron_step(sp["n"], len(csr[2]), sp["T_mem"]))
     plans = [r.kernel_plan() for r in res]
-    groups, singles = {}, []
-    for i, pl in enumerate(plans):
-        (groups.setdefault(pl["nb"], []) if pl["kernel"] == 2 else singles).append(i)
-    launches = []
-    for nb, idx in sorted(groups.items()):
-        # launch-mates finish together when their step times are alike: order by
-        # row-pull kind, history length and size
-        idx.sort(key=lambda i: (plans[i]["weights_lds"], plans[i]["row_pull"], specs[i]["T_mem"], -specs[i]["n"]))
-        cur = []
-        for i in idx:
-            if cur and sa.batch_workgroups([res[j] for j in cur + [i]]) == 0:
-                launches.append(cur)
-                cur = []
-            cur.append(i)
-        if cur:
-            launches.append(cur)
-    launches += [[i] for i in singles]
+    tp = [i for i in range(len(res)) if plans[i]["kernel"] == 2]
+    # launch-mates finish together when their step times are alike: order by
+    # row-pull kind, history length and size
+    tp.sort(key=lambda i: (plans[i]["weights_lds"], plans[i]["row_pull"], specs[i]["T_mem"], -specs[i]["n"]))
+    singles = [i for i in range(len(res)) if plans[i]["kernel"] != 2]
     total_n = sum(sp["n"] for sp in specs)
     flags = sa.REC_STATE | sa.REC_APPEND
 
-    def block(t, K):
-        for grp in launches:
-            if len(grp) > 1:
-                sa.run_resident_batch([res[i] for i in grp], t, K, flags)
-            else:
-                res[grp[0]].run_resident(t, K, flags)
+    def eager_block(t, K):
+        """The same launches without the graph (run_resident_batch per launch)."""
+        cur, wgs = [], 0
+        groups = []
+        for nb in (32, 16):
+            cur, wgs = [], 0
+            for i in tp:
+                if plans[i]["nb"] != nb:
+                    continue
+                if cur and wgs + plans[i]["workgroups"] > 256:
+                    groups.append(cur)
+                    cur, wgs = [], 0
+                cur.append(i)
+                wgs += plans[i]["workgroups"]
+            if cur:
+                groups.append(cur)
+        for g in groups:
+            sa.run_resident_batch([res[i] for i in g], t, K, flags)
 
-    # untimed warm-up until every history is full, then K autotune on the next
-    # blocks of the run (each candidate advances every reservoir)
-    t, ktimes = 1024, {}
-    block(0, t)
+    def single_block(t, K):
+        for i in singles:
+            res[i].run_resident(t, K, flags)
+
+    # untimed warm-up until every history is full
+    t = 1024
+    tb = sa.TimeBlock([res[i] for i in tp], t, sa.REC_STATE)
+    tb.run(1)
+    tb.close()
+    single_block(0, t)
+    # K autotune: one graph block per candidate (each advances every reservoir)
+    ktimes = {}
     for K in (64, 256, 1024):
+        tb = sa.TimeBlock([res[i] for i in tp], K, sa.REC_STATE)
         t0 = time.perf_counter()
-        block(t, K)
+        tb.run(1)
+        single_block(t, K)
         ktimes[K] = (time.perf_counter() - t0) / K * 1e6
+        tb.close()
         t += K
     K = min(ktimes, key=ktimes.get)
-    steps = T - t
+    # eager launches for comparison: two blocks
     t0 = time.perf_counter()
-    while t < T:
-        k = min(K, T - t)
-        block(t, k)
-        t += k
+    for _ in range(2):
+        eager_block(t, K)
+        single_block(t, K)
+        t += K
+    eager_us = (time.perf_counter() - t0) / (2 * K) * 1e6
+    # timed: graph replays of whole blocks
+    tb = sa.TimeBlock([res[i] for i in tp], K, sa.REC_STATE)
+    launches = tb.launches + len(singles)
+    blocks = (T - t) // K
+    t0 = time.perf_counter()
+    for _ in range(blocks):
+        tb.run(1)
+        single_block(t, K)
+        t += K
     wall = time.perf_counter() - t0
+    tb.close()
+    steps = blocks * K
     value = total_n * steps / wall
     alg = sum(b * sp["n"] for b, sp in zip(bps, specs)) * steps / wall / 1e9
     for r in res:
@@ -309,14 +335,14 @@ def run_batched(args):
         "config": {"workload": "64 FLIF-GL reservoirs, N log-U[256,8192] (total %d neurons), ER p=0.05 / BA m=5, "
                                "alpha U[0.5,1], T_mem in {128,256,512,1024}, rho U[0.7,1.2], N(0,1) weights; "
                                "13-channel AR(1) class sequences (512 sequences, %d frames)" % (total_n, T),
-                   "block_k": K},
+                   "block_k": K, "graph": "HIP graph per time block (%d product-wave reservoirs)" % len(tp)},
         "k_autotune_us_per_step": ktimes,
-        "launches_per_block": len(launches),
-        "batched_launches": [[specs[i]["n"] for i in g] for g in launches if len(g) > 1],
+        "eager_us_per_step": eager_us,
+        "launches_per_block": launches,
         "single_launches": [dict(n=specs[i]["n"], T_mem=specs[i]["T_mem"], kernel=plans[i]["kernel"]) for i in singles],
         "ms_per_step": wall / steps * 1e3,
         "roofline": {"bound": "hbm", "achieved": alg, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": alg / HBM_PEAK_GBS,
-                     "traffic": None, "kernel": "k_persist_tp_batch + k_persist (all launches of a block)"},
+                     "traffic": None, "kernel": "k_persist_tp_batch (graph) + k_persist (all launches of a block)"},
     }
     print(json.dumps(line), flush=True)
 

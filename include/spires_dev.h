@@ -95,6 +95,20 @@ int  spires_dev_run(spires_dev *g, size_t t0, size_t T, unsigned flags);
 int  spires_dev_run_batch(spires_dev *const *gs, int count, size_t t0, size_t T, unsigned flags);
 /* Workgroups a batch would use (0 if the reservoirs cannot run batched). */
 int  spires_dev_batch_workgroups(spires_dev *const *gs, int count);
+/* Time blocks replayed from a HIP graph: the launches that advance `count`
+ * exact-mode FLIF-GL reservoirs (product-wave kernel) by K macro steps,
+ * packed per neurons-per-workgroup class into launches of at most one
+ * workgroup per CU, captured once with device-side step clocks; each replay
+ * runs the next K steps of every reservoir.  Inputs are read at row
+ * msteps + k; with REC_STATE / REC_RASTER, records are written at row
+ * msteps + k (reserve them beforehand; reallocating inputs or records
+ * invalidates the time block). */
+typedef struct spires_dev_timeblock spires_dev_timeblock;
+int  spires_dev_timeblock_create(spires_dev *const *gs, int count, long long K, unsigned flags,
+                                 spires_dev_timeblock **out);
+int  spires_dev_timeblock_run(spires_dev_timeblock *tb, long long blocks);
+int  spires_dev_timeblock_launches(const spires_dev_timeblock *tb);
+void spires_dev_timeblock_destroy(spires_dev_timeblock *tb);
 /* Exact-mode launch plan: info[0] kernel (0 per-step launches, 1 single-wave
  * persistent, 2 product-wave persistent), [1] neurons per workgroup,
  * [2] workgroups, [3] row pull (0 edge list, 1 spike-word bitmaps),

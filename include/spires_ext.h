@@ -94,6 +94,20 @@ spires_status spires_ext_fit_ridge_records_batch(spires_reservoir *const *rs, in
 spires_status spires_ext_run_resident_batch(spires_reservoir *const *rs, int count, size_t t0, size_t T,
                                             unsigned flags);
 int           spires_ext_batch_workgroups(spires_reservoir *const *rs, int count);
+/* Time blocks under a HIP graph: one graph holds the launches that advance
+ * `count` exact-mode FLIF-GL reservoirs (product-wave kernel) by K macro
+ * steps; every spires_ext_timeblock_run replays it `blocks` times.  Step
+ * counters live on the device, so each replay continues where the last one
+ * stopped.  Inputs are read, and records (flags SPIRES_EXT_REC_STATE /
+ * _RASTER) written, at row = macro step since reset; reserve the records
+ * first.  No counterpart in the reference. */
+typedef struct spires_timeblock spires_timeblock;
+spires_status spires_ext_timeblock_create(spires_reservoir *const *rs, int count, size_t K, unsigned flags,
+                                          spires_timeblock **out);
+spires_status spires_ext_timeblock_run(spires_timeblock *tb, size_t blocks);
+int           spires_ext_timeblock_launches(const spires_timeblock *tb);
+void          spires_ext_timeblock_destroy(spires_timeblock *tb);
+
 /* The exact-mode launch plan of a reservoir (spires_dev_plan_info layout:
  * kernel, neurons per workgroup, workgroups, row-pull kind, weights in LDS,
  * LDS bytes).  No counterpart in the reference. */

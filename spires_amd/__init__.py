@@ -106,6 +106,10 @@ SIGNATURES = {
     "spires_ext_fit_ridge_records": (C.c_int, [_P, _SZ, _SZ, _D, C.c_double, _D]),
     "spires_ext_batch_workgroups": (C.c_int, [C.POINTER(_P), C.c_int]),
     "spires_ext_kernel_plan": (C.c_int, [_P, C.POINTER(C.c_int)]),
+    "spires_ext_timeblock_create": (C.c_int, [C.POINTER(_P), C.c_int, _SZ, C.c_uint, C.POINTER(_P)]),
+    "spires_ext_timeblock_run": (C.c_int, [_P, _SZ]),
+    "spires_ext_timeblock_launches": (C.c_int, [_P]),
+    "spires_ext_timeblock_destroy": (None, [_P]),
     "spires_ext_predict_records": (C.c_int, [_P, _SZ, _SZ, _D]),
     "spires_ext_fit_ridge_records_batch": (C.c_int, [C.POINTER(_P), C.c_int, _SZ, _SZ, C.POINTER(_D), _D, _D]),
     "spires_ext_train_ridge_washout": (C.c_int, [_P, _D, _D, _SZ, _SZ, C.c_double, _D]),
@@ -257,6 +261,34 @@ def batch_workgroups(reservoirs):
     """Workgroups a batched run of these reservoirs would use (0: cannot batch)."""
     arr = (_P * len(reservoirs))(*[r.h for r in reservoirs])
     return lib().spires_ext_batch_workgroups(arr, len(reservoirs))
+
+
+class TimeBlock:
+    """K-step time blocks of several exact-mode FLIF-GL reservoirs replayed from
+    one HIP graph (spires_ext_timeblock_*).  Inputs are read and records written
+    at row = macro step since reset; reserve the records before creating it."""
+
+    def __init__(self, reservoirs, K, flags=None):
+        self._res = list(reservoirs)
+        flags = REC_STATE if flags is None else flags
+        arr = (_P * len(self._res))(*[r.h for r in self._res])
+        self.h = _P()
+        _check(lib().spires_ext_timeblock_create(arr, len(self._res), K, flags, C.byref(self.h)))
+        self.K = K
+
+    @property
+    def launches(self):
+        return lib().spires_ext_timeblock_launches(self.h)
+
+    def run(self, blocks=1):
+        _check(lib().spires_ext_timeblock_run(self.h, blocks))
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().spires_ext_timeblock_destroy(self.h)
+            self.h = None
+
+    __del__ = close
 
 
 class Surrogate:

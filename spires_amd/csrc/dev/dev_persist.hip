@@ -55,6 +55,10 @@ struct PersistArgs {
     unsigned* abort_flag;
     double* rec_V; uint8_t* rec_S;
     const unsigned* rbm;          // [n][ceil(n/32)] row bitmaps in column order (rpm 1)
+    // Device step clock (replayed time blocks, k_persist_tp only): when set,
+    // st0 = clk[0], t0 = clk[1], spikes are read from S (clk[2] even) or S_out
+    // (odd) and written to the other, and records start at row t0.
+    const long long* clk;
 };
 
 // FP64 multiply and add with their issue order pinned: each is one IEEE
@@ -445,11 +449,29 @@ __device__ __forceinline__ void tp_body(const PersistArgs& a, const int g) {
     __shared__ int abort_s;
     __shared__ unsigned cons_s;
     if (tid == 0) { abort_s = 0; cons_s = 0u; }
+    long long st0 = a.st0, t0 = a.t0;
+    const uint8_t* S_in = a.S;
+    uint8_t* S_out = a.S_out;
+    double* rec_V = a.rec_V;
+    uint8_t* rec_S = a.rec_S;
+    if (a.clk) {
+        const long long c0 = a.clk[0], c1 = a.clk[1], c2 = a.clk[2];
+        st0 = (long long)(unsigned)__builtin_amdgcn_readfirstlane((int)c0) |
+              ((long long)__builtin_amdgcn_readfirstlane((int)(c0 >> 32)) << 32);
+        t0 = (long long)(unsigned)__builtin_amdgcn_readfirstlane((int)c1) |
+             ((long long)__builtin_amdgcn_readfirstlane((int)(c1 >> 32)) << 32);
+        if (__builtin_amdgcn_readfirstlane((int)c2) & 1) {
+            S_in = a.S_out;
+            S_out = const_cast<uint8_t*>(a.S);
+        }
+        if (rec_V) rec_V += (size_t)t0 * n;
+        if (rec_S) rec_S += (size_t)t0 * n;
+    }
     for (int k = tid; k < PS; k += kTpThreads) coef_s[k] = k < L ? a.coef[k] : 0.0;
-    const int head0 = (int)(a.st0 % L);
+    const int head0 = (int)(st0 % L);
     {
         const int sp = head0 == 0 ? L - 1 : head0 - 1;   // V_{st0-1}
-        const int slot = (int)((a.st0 - 1) & 1);
+        const int slot = (int)((st0 - 1) & 1);
         for (int e = tid; e < NB; e += kTpThreads)
             stage[slot * NB + e] = e < nloc ? a.ring[(size_t)sp * n + i0 + e] : 0.0;
     }
@@ -486,7 +508,7 @@ __device__ __forceinline__ void tp_body(const PersistArgs& a, const int g) {
     const int i = i0 + lane;
     double V = own ? a.V[i] : 0.0;
     double tp = own ? a.tprev[i] : 0.0;
-    bool spk = own ? (a.S[i] != 0) : false;
+    bool spk = own ? (S_in[i] != 0) : false;
     int rs = 0, re = 0;
     constexpr int kMaxDinReg = 4;
     double win[kMaxDinReg];
@@ -509,7 +531,7 @@ __device__ __forceinline__ void tp_body(const PersistArgs& a, const int g) {
     const int pw = wave == 5 ? 3 : wave - 1;
     const int part = pw * (64 / NB) + lane / NB;
     int head = head0;
-    int lim = a.st0 < L ? (int)a.st0 : L - 1;
+    int lim = st0 < L ? (int)st0 : L - 1;
     // products of the launch's first step, all from the HBM ring
     if (producer) {
         const int nk = lim >= 2 ? lim - 1 : 0;
@@ -529,10 +551,10 @@ __device__ __forceinline__ void tp_body(const PersistArgs& a, const int g) {
 
     const double c1 = coef_s[1];
     const double2* prow = reinterpret_cast<const double2*>(P + (size_t)r * PS);   // pair j: lags 2j, 2j+1
-    long long st = a.st0;
+    long long st = st0;
     unsigned m = 0;                                    // micro-step index within the launch
     for (long long t = 0; t < a.T; ++t) {
-        const long long tt = a.t0 + t;
+        const long long tt = t0 + t;
         const int tb = (int)(t & 1);
         for (int mi = 0; mi < a.micro; ++mi, ++st, ++m) {
             const int sb = (int)(st & 1);
@@ -596,8 +618,8 @@ __device__ __forceinline__ void tp_body(const PersistArgs& a, const int g) {
                     stage[sb * NB + lane] = V;
                     tp = tp + a.dt;
                     if (mi == a.micro - 1) {
-                        if (a.rec_V) a.rec_V[(size_t)t * n + i] = V;
-                        if (a.rec_S) a.rec_S[(size_t)t * n + i] = spk ? 1 : 0;
+                        if (rec_V) rec_V[(size_t)t * n + i] = V;
+                        if (rec_S) rec_S[(size_t)t * n + i] = spk ? 1 : 0;
                     }
                 }
                 lds_counter_store(&cons_s, (m + 1u) * 64u);   // tail done: V_st staged
@@ -608,7 +630,7 @@ __device__ __forceinline__ void tp_body(const PersistArgs& a, const int g) {
 #pragma unroll
                     for (int j = 0; j < kMaxDinReg; ++j) uin[j] = (j < a.din) ? a.u[tt * a.din + j] : 0.0;
                 }
-                const bool aborted = !gather_spikes(a.xch, nw, NB, wpg, n, st, a.st0, a.S, bits_s,
+                const bool aborted = !gather_spikes(a.xch, nw, NB, wpg, n, st, st0, S_in, bits_s,
                                                     a.abort_flag, &abort_s, lane);
                 __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
                 __builtin_amdgcn_wave_barrier();
@@ -724,7 +746,7 @@ __device__ __forceinline__ void tp_body(const PersistArgs& a, const int g) {
     }
     if (own) {
         a.V[i] = V;
-        a.S_out[i] = spk ? 1 : 0;
+        S_out[i] = spk ? 1 : 0;
         a.tprev[i] = tp;
     }
 }
@@ -1042,5 +1064,233 @@ extern "C" int spires_dev_plan_info(spires_dev* g, int* info) {
     info[3] = pl.rpm;
     info[4] = pl.ne_lds > 0 ? 1 : 0;
     info[5] = (int)pl.lds;
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Time blocks replayed from a HIP graph.  The launches of one block of K macro
+// steps (product-wave batches, packed per neurons-per-workgroup class) are
+// captured once; each kernel reads its reservoirs' step counters from a
+// device clock, and a one-thread-per-reservoir kernel after it advances the
+// clocks, so every replay runs the next K steps.  Abort flags are folded into
+// a sticky count by that kernel and checked after the replays.
+
+namespace sdev {
+
+struct TbItem {
+    long long* clk;       // [4]: micro-step, macro step, spike-buffer parity, unused
+    unsigned* abort_flag;
+    unsigned* sticky;
+    long long steps;
+    int micro;
+};
+
+__global__ void k_timeblock_advance(const TbItem* __restrict__ items, int count) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= count) return;
+    const TbItem it = items[r];
+    it.clk[0] += it.steps * it.micro;
+    it.clk[1] += it.steps;
+    it.clk[2] ^= 1;
+    *it.sticky += *it.abort_flag;
+    *it.abort_flag = 0u;
+}
+
+}  // namespace sdev
+
+struct spires_dev_timeblock {
+    std::vector<spires_dev*> gs;     // in launch order
+    long long K = 0;
+    unsigned flags = 0;
+    int launches = 0;
+    hipStream_t stream = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    sdev::PersistArgs* table = nullptr;
+    int2* map = nullptr;
+    sdev::TbItem* items = nullptr;
+    long long* clk = nullptr;
+    unsigned* sticky = nullptr;
+    std::vector<const double*> u_at;
+    std::vector<double*> recv_at;
+    std::vector<uint8_t*> recs_at;
+};
+
+extern "C" void spires_dev_timeblock_destroy(spires_dev_timeblock* tb) {
+    if (!tb) return;
+    if (tb->stream) (void)hipStreamSynchronize(tb->stream);
+    if (tb->exec) (void)hipGraphExecDestroy(tb->exec);
+    if (tb->graph) (void)hipGraphDestroy(tb->graph);
+    sdev::dfree(tb->table); sdev::dfree(tb->map); sdev::dfree(tb->items); sdev::dfree(tb->clk); sdev::dfree(tb->sticky);
+    if (tb->stream) (void)hipStreamDestroy(tb->stream);
+    delete tb;
+}
+
+extern "C" int spires_dev_timeblock_create(spires_dev* const* gs, int count, long long K, unsigned flags,
+                                           spires_dev_timeblock** out) {
+    using namespace sdev;
+    if (!gs || count < 1 || K < 1 || !out) return SPIRES_DEV_EINVAL;
+    *out = nullptr;
+    if (flags & ~(SPIRES_DEV_REC_STATE | SPIRES_DEV_REC_RASTER | SPIRES_DEV_REC_APPEND)) {
+        set_error("time blocks record states and rasters only");
+        return SPIRES_DEV_EINVAL;
+    }
+    const int ncu = num_cus();
+    std::vector<Plan> plans(count);
+    for (int r = 0; r < count; ++r) {
+        spires_dev* g = gs[r];
+        if (!g) return SPIRES_DEV_EINVAL;
+        if (g->mode != SPIRES_DEV_MODE_EXACT || g->path == 1 || !persist_plan(g, ncu, &plans[r]) || !plans[r].tp) {
+            set_error("time blocks need exact-mode FLIF-GL reservoirs that fit the product-wave kernel");
+            return SPIRES_DEV_EINVAL;
+        }
+        if (g->din > 0 && !g->u_series) { set_error("time blocks need uploaded inputs"); return SPIRES_DEV_EINVAL; }
+        if ((flags & SPIRES_DEV_REC_STATE) && !g->rec_state) { set_error("reserve state records before creating a time block"); return SPIRES_DEV_EINVAL; }
+        if ((flags & SPIRES_DEV_REC_RASTER) && !g->rec_raster) { set_error("reserve raster records before creating a time block"); return SPIRES_DEV_EINVAL; }
+        for (int q = 0; q < r; ++q)
+            if (gs[q] == g) { set_error("a reservoir appears twice in a time block"); return SPIRES_DEV_EINVAL; }
+    }
+    // pack launches: per neurons-per-workgroup class, greedily up to one workgroup per CU
+    std::vector<std::vector<int>> launches;
+    for (int nb : {32, 16}) {
+        std::vector<int> cur;
+        int wgs = 0;
+        for (int r = 0; r < count; ++r) {
+            if (plans[r].nb != nb) continue;
+            if (!cur.empty() && wgs + plans[r].G > ncu) { launches.push_back(cur); cur.clear(); wgs = 0; }
+            cur.push_back(r);
+            wgs += plans[r].G;
+        }
+        if (!cur.empty()) launches.push_back(cur);
+    }
+    spires_dev_timeblock* tb = new spires_dev_timeblock();
+    tb->K = K;
+    tb->flags = flags;
+    tb->launches = (int)launches.size();
+    auto fail = [&](int rc) { spires_dev_timeblock_destroy(tb); return rc; };
+    std::vector<PersistArgs> table;
+    std::vector<int2> map;
+    std::vector<TbItem> items;
+    std::vector<size_t> map_off, item_off, lds;
+    std::vector<int> nbs;
+    size_t lds32 = 0, lds16 = 0;
+    int total = 0;
+    for (auto& L : launches) total += (int)L.size();
+    int rc = 0;
+    if ((rc = dalloc(&tb->clk, (size_t)total * 4)) || (rc = dalloc(&tb->sticky, (size_t)total))) return fail(rc);
+    for (auto& L : launches) {
+        map_off.push_back(map.size());
+        item_off.push_back(items.size());
+        size_t l = 0;
+        for (int r : L) {
+            spires_dev* g = gs[r];
+            const Plan& pl = plans[r];
+            if (!g->ring64_valid) {
+                if ((rc = fast_sync_rings(g, false))) return fail(rc);
+                g->ring64_valid = 1;
+            }
+            g->ring32_valid = 0;
+            if ((rc = prepare_launch(g, pl))) return fail(rc);
+            const int idx = (int)table.size();
+            PersistArgs a = make_args(g, pl, g->din > 0 ? g->u_series : nullptr, 0, K,
+                                      (flags & SPIRES_DEV_REC_STATE) ? g->rec_state : nullptr,
+                                      (flags & SPIRES_DEV_REC_RASTER) ? g->rec_raster : nullptr);
+            a.S = g->S[0];
+            a.S_out = g->S[1];
+            a.clk = tb->clk + 4 * idx;
+            table.push_back(a);
+            for (int b = 0; b < pl.G; ++b) map.push_back(make_int2(idx, b));
+            items.push_back(TbItem{tb->clk + 4 * idx, g->abort_flag, tb->sticky + idx, K, g->micro});
+            tb->gs.push_back(g);
+            tb->u_at.push_back(g->u_series);
+            tb->recv_at.push_back(g->rec_state);
+            tb->recs_at.push_back(g->rec_raster);
+            l = std::max(l, pl.lds);
+        }
+        lds.push_back(l);
+        const int nb = plans[L[0]].nb;
+        nbs.push_back(nb);
+        (nb == 32 ? lds32 : lds16) = std::max(nb == 32 ? lds32 : lds16, l);
+    }
+    if ((rc = dalloc(&tb->table, table.size())) || (rc = dalloc(&tb->map, map.size())) ||
+        (rc = dalloc(&tb->items, items.size())))
+        return fail(rc);
+    hipError_t e = hipStreamCreate(&tb->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(tb->table, table.data(), table.size() * sizeof(PersistArgs), hipMemcpyHostToDevice, tb->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(tb->map, map.data(), map.size() * sizeof(int2), hipMemcpyHostToDevice, tb->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(tb->items, items.data(), items.size() * sizeof(TbItem), hipMemcpyHostToDevice, tb->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(tb->stream);
+    if (e == hipSuccess && lds32) e = hipFuncSetAttribute((const void*)k_persist_tp_batch<32>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds32);
+    if (e == hipSuccess && lds16) e = hipFuncSetAttribute((const void*)k_persist_tp_batch<16>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds16);
+    if (e != hipSuccess) { set_error(hipGetErrorString(e)); return fail(SPIRES_DEV_EINTERNAL); }
+    // capture one block: each launch, then the clock advance of its reservoirs
+    e = hipStreamBeginCapture(tb->stream, hipStreamCaptureModeThreadLocal);
+    for (size_t q = 0; q < launches.size() && e == hipSuccess; ++q) {
+        const int G = (int)((q + 1 < launches.size() ? map_off[q + 1] : map.size()) - map_off[q]);
+        const int cnt = (int)launches[q].size();
+        if (nbs[q] == 32)
+            hipLaunchKernelGGL(k_persist_tp_batch<32>, dim3(G), dim3(kTpThreads), (unsigned)lds[q], tb->stream,
+                               (const PersistArgs*)tb->table, (const int2*)(tb->map + map_off[q]));
+        else
+            hipLaunchKernelGGL(k_persist_tp_batch<16>, dim3(G), dim3(kTpThreads), (unsigned)lds[q], tb->stream,
+                               (const PersistArgs*)tb->table, (const int2*)(tb->map + map_off[q]));
+        hipLaunchKernelGGL(k_timeblock_advance, dim3(1), dim3(64 * ((cnt + 63) / 64)), 0, tb->stream,
+                           (const TbItem*)(tb->items + item_off[q]), cnt);
+        e = hipGetLastError();
+    }
+    hipError_t e2 = hipStreamEndCapture(tb->stream, &tb->graph);
+    if (e == hipSuccess) e = e2;
+    if (e == hipSuccess) e = hipGraphInstantiate(&tb->exec, tb->graph, nullptr, nullptr, 0);
+    if (e != hipSuccess) { set_error(std::string("time block capture: ") + hipGetErrorString(e)); return fail(SPIRES_DEV_EINTERNAL); }
+    *out = tb;
+    return 0;
+}
+
+extern "C" int spires_dev_timeblock_launches(const spires_dev_timeblock* tb) { return tb ? tb->launches : 0; }
+
+extern "C" int spires_dev_timeblock_run(spires_dev_timeblock* tb, long long blocks) {
+    using namespace sdev;
+    if (!tb || blocks < 0) return SPIRES_DEV_EINVAL;
+    if (blocks == 0) return 0;
+    const size_t cnt = tb->gs.size();
+    const long long steps = blocks * tb->K;
+    std::vector<long long> clk(cnt * 4, 0);
+    for (size_t r = 0; r < cnt; ++r) {
+        spires_dev* g = tb->gs[r];
+        if (g->u_series != tb->u_at[r] || g->rec_state != tb->recv_at[r] || g->rec_raster != tb->recs_at[r]) {
+            set_error("inputs or records were reallocated after the time block was made");
+            return SPIRES_DEV_EINVAL;
+        }
+        if (g->mode != SPIRES_DEV_MODE_EXACT) { set_error("time blocks run in exact mode"); return SPIRES_DEV_EINVAL; }
+        const size_t end = g->msteps + (size_t)steps;
+        if (g->din > 0 && end > g->u_T) { set_error("run beyond uploaded inputs"); return SPIRES_DEV_EINVAL; }
+        if (((tb->flags & SPIRES_DEV_REC_STATE) && end * g->n > g->rec_cap_s) ||
+            ((tb->flags & SPIRES_DEV_REC_RASTER) && end * g->n > g->rec_cap_r)) {
+            set_error("records too small for the time blocks (rows = macro step index)");
+            return SPIRES_DEV_EINVAL;
+        }
+        if (!g->ring64_valid) { set_error("the FP64 history is stale (fast-mode run after the time block was made)"); return SPIRES_DEV_EINVAL; }
+        SDEV_CK(hipStreamSynchronize(g->stream));
+        clk[4 * r + 0] = g->step;
+        clk[4 * r + 1] = (long long)g->msteps;
+        clk[4 * r + 2] = g->cur;
+    }
+    SDEV_CK(hipMemcpyAsync(tb->clk, clk.data(), clk.size() * 8, hipMemcpyHostToDevice, tb->stream));
+    SDEV_CK(hipMemsetAsync(tb->sticky, 0, cnt * 4, tb->stream));
+    for (spires_dev* g : tb->gs) SDEV_CK(hipMemsetAsync(g->abort_flag, 0, 4, tb->stream));
+    for (long long b = 0; b < blocks; ++b) SDEV_CK(hipGraphLaunch(tb->exec, tb->stream));
+    std::vector<unsigned> sticky(cnt, 0);
+    SDEV_CK(hipMemcpyAsync(sticky.data(), tb->sticky, cnt * 4, hipMemcpyDeviceToHost, tb->stream));
+    SDEV_CK(hipStreamSynchronize(tb->stream));
+    for (size_t r = 0; r < cnt; ++r) {
+        spires_dev* g = tb->gs[r];
+        g->step += steps * g->micro;
+        g->msteps += (size_t)steps;
+        g->cur ^= (int)(blocks & 1);
+        g->ring32_valid = 0;
+        if (tb->flags & (SPIRES_DEV_REC_STATE | SPIRES_DEV_REC_RASTER)) g->rec_rows = std::max(g->rec_rows, g->msteps);
+    }
+    for (size_t r = 0; r < cnt; ++r)
+        if (sticky[r]) { set_error("a time-block kernel timed out waiting for the spike exchange"); return SPIRES_DEV_EINTERNAL; }
     return 0;
 }

@@ -654,6 +654,40 @@ spires_status spires_ext_run_resident_batch(spires_reservoir* const* rs, int cou
     return dev_status(spires_dev_run_batch(devs.data(), count, t0, T, flags));
 }
 
+struct spires_timeblock {
+    spires_dev_timeblock* tb = nullptr;
+};
+
+spires_status spires_ext_timeblock_create(spires_reservoir* const* rs, int count, size_t K, unsigned flags,
+                                          spires_timeblock** out) {
+    if (!rs || count < 1 || K < 1 || !out) return SPIRES_ERR_INVALID_ARG;
+    *out = nullptr;
+    std::vector<spires_dev*> devs(count);
+    for (int i = 0; i < count; ++i) {
+        if (!rs[i] || !rs[i]->dev) return SPIRES_ERR_INVALID_ARG;
+        devs[i] = rs[i]->dev;
+    }
+    spires_timeblock* t = new (std::nothrow) spires_timeblock();
+    if (!t) return SPIRES_ERR_ALLOC;
+    spires_status s = dev_status(spires_dev_timeblock_create(devs.data(), count, (long long)K, flags, &t->tb));
+    if (s != SPIRES_OK) { delete t; return s; }
+    *out = t;
+    return SPIRES_OK;
+}
+
+spires_status spires_ext_timeblock_run(spires_timeblock* tb, size_t blocks) {
+    if (!tb || !tb->tb) return SPIRES_ERR_INVALID_ARG;
+    return dev_status(spires_dev_timeblock_run(tb->tb, (long long)blocks));
+}
+
+int spires_ext_timeblock_launches(const spires_timeblock* tb) { return tb ? spires_dev_timeblock_launches(tb->tb) : 0; }
+
+void spires_ext_timeblock_destroy(spires_timeblock* tb) {
+    if (!tb) return;
+    spires_dev_timeblock_destroy(tb->tb);
+    delete tb;
+}
+
 spires_status spires_ext_kernel_plan(spires_reservoir* r, int* info) {
     if (!r || !r->dev || !info) return SPIRES_ERR_INVALID_ARG;
     return dev_status(spires_dev_plan_info(r->dev, info));
