@@ -326,7 +326,7 @@ def _levy_steps(rng, count, dim, mu=1.3, lo=1e-3, hi=0.3):
 
 def surrogate_search(ev, rungs, target=None, budget=512, warm=96, wave=32, promote=8, pool=2048, seed=1,
                      epochs=3000, round_epochs=500, lr=1e-3, reseed=4, explore=8, top_weight=4, clip=2.0,
-                     log=None):
+                     novelty=0.5, log=None):
     """Surrogate-guided search with the fidelity rungs.
 
     Every trial first runs to the middle rung (the lowest rung is a checkpoint
@@ -341,8 +341,8 @@ def surrogate_search(ev, rungs, target=None, budget=512, warm=96, wave=32, promo
     trials from a Latin hypercube; each later wave holds `wave` proposals:
       - `reseed` fresh weight seeds for the two best hyperparameter sets so far
         (a trial's score varies strongly with its weight draw);
-      - `explore` global samples, half unscreened and half the surrogate's
-        best-ranked of `pool`;
+      - `explore` global samples, half unscreened and half the best of `pool`
+        by predicted score minus `novelty` x distance to the nearest trial run;
       - the rest from its ranking of `pool` Levy-flight moves around the eight
         best top-rung trials.
     Every reported score comes from a full run; the search stops when a
@@ -415,8 +415,14 @@ def surrogate_search(ev, rungs, target=None, budget=512, warm=96, wave=32, promo
         pick = [inc[q % min(2, len(inc))][0] for q in range(reseed)]
         glob = sample_trials(pool, int(rng.integers(1 << 31)))
         pg = net.predict(encode(glob, f_max, f_min, f_max))
+        # screened global picks favour unexplored regions: predicted score minus
+        # `novelty` x distance (unit cube) to the nearest trial run so far
+        seen = to_unit(np.array([h[0] for h in history]))
+        gz = to_unit(glob)
+        dist = np.sqrt(((gz[:, None, :] - seen[None, :, :]) ** 2).sum(axis=2)).min(axis=1)
+        acq = pg - novelty * dist
         pick += list(glob[:explore // 2])                                   # unscreened
-        pick += [glob[i] for i in np.argsort(pg) if i >= explore // 2][:explore - explore // 2]
+        pick += [glob[i] for i in np.argsort(acq) if i >= explore // 2][:explore - explore // 2]
         base = to_unit(np.array([d[0] for d in inc[:8]]))
         loc = from_unit(base[rng.integers(len(base), size=pool)] + _levy_steps(rng, pool, 6, hi=0.15))
         pl = net.predict(encode(loc, f_max, f_min, f_max))
