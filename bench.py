@@ -257,11 +257,11 @@ def run_batched(args):
         res.append(r)
         bps.append(workloads.algorithmic_bytes_per_neuron_step(sp["n"], len(csr[2]), sp["T_mem"]))
     plans = [r.kernel_plan() for r in res]
-    tp = [i for i in range(len(res)) if plans[i]["kernel"] == 2]
+    tp = [i for i in range(len(res)) if plans[i]["batch_nb"] > 0]
     # launch-mates finish together when their step times are alike: order by
     # row-pull kind, history length and size
     tp.sort(key=lambda i: (plans[i]["weights_lds"], plans[i]["row_pull"], specs[i]["T_mem"], -specs[i]["n"]))
-    singles = [i for i in range(len(res)) if plans[i]["kernel"] != 2]
+    singles = [i for i in range(len(res)) if plans[i]["batch_nb"] == 0]
     total_n = sum(sp["n"] for sp in specs)
     flags = sa.REC_STATE | sa.REC_APPEND
 
@@ -272,13 +272,14 @@ def run_batched(args):
         for nb in (32, 16):
             cur, wgs = [], 0
             for i in tp:
-                if plans[i]["nb"] != nb:
+                if plans[i]["batch_nb"] != nb:
                     continue
-                if cur and wgs + plans[i]["workgroups"] > 256:
+                w = -(-specs[i]["n"] // nb)
+                if cur and sa.batch_workgroups([res[j] for j in cur + [i]]) == 0:
                     groups.append(cur)
                     cur, wgs = [], 0
                 cur.append(i)
-                wgs += plans[i]["workgroups"]
+                wgs += w
             if cur:
                 groups.append(cur)
         for g in groups:
@@ -491,6 +492,7 @@ def main():
     n = cfg[0].num_neurons
     L = 512
     r = sa.Reservoir(cfg, csr=csr, W_in=W_in)
+    plan = r.kernel_plan()
     r.upload_inputs(u)
     r.reserve_records(max(K, W), sa.REC_STATE)
     ev = HipEvents()
@@ -534,7 +536,7 @@ def main():
                      "traffic": traffic_step * K if traffic_step else None,
                      "traffic_source": traffic_src,
                      "algorithmic_bytes_per_launch": bps * n * K,
-                     "kernel": "k_persist_tp<32> (one launch of K steps)"},
+                     "kernel": "k_persist_tp<%d> (one launch of K steps, %d workgroups)" % (plan["nb"], plan["workgroups"])},
         "wall_s": wall,
         "final_state_checksum": float(np.sum(X)),
     }

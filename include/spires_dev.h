@@ -109,10 +109,11 @@ int  spires_dev_timeblock_create(spires_dev *const *gs, int count, long long K, 
 int  spires_dev_timeblock_run(spires_dev_timeblock *tb, long long blocks);
 int  spires_dev_timeblock_launches(const spires_dev_timeblock *tb);
 void spires_dev_timeblock_destroy(spires_dev_timeblock *tb);
-/* Exact-mode launch plan: info[0] kernel (0 per-step launches, 1 single-wave
- * persistent, 2 product-wave persistent), [1] neurons per workgroup,
- * [2] workgroups, [3] row pull (0 edge list, 1 spike-word bitmaps),
- * [4] recurrent weights held in LDS, [5] LDS bytes per workgroup. */
+/* Exact-mode launch plan of a run of this reservoir alone: info[0] kernel
+ * (0 per-step launches, 1 single-wave persistent, 2 product-wave persistent),
+ * [1] neurons per workgroup, [2] workgroups, [3] row pull (0 edge list,
+ * 1 column bitmaps), [4] recurrent weights held in LDS, [5] LDS bytes per
+ * workgroup; [6] neurons per workgroup in a batched launch (0: cannot batch). */
 int  spires_dev_plan_info(spires_dev *g, int *info);
 /* Pre-size the record buffers (so a later run allocates nothing). */
 int  spires_dev_reserve_records(spires_dev *g, size_t rows, unsigned flags);

@@ -108,9 +108,10 @@ spires_status spires_ext_timeblock_run(spires_timeblock *tb, size_t blocks);
 int           spires_ext_timeblock_launches(const spires_timeblock *tb);
 void          spires_ext_timeblock_destroy(spires_timeblock *tb);
 
-/* The exact-mode launch plan of a reservoir (spires_dev_plan_info layout:
- * kernel, neurons per workgroup, workgroups, row-pull kind, weights in LDS,
- * LDS bytes).  No counterpart in the reference. */
+/* The exact-mode launch plan of a reservoir, info[7] (spires_dev_plan_info
+ * layout: kernel, neurons per workgroup, workgroups, row-pull kind, weights
+ * in LDS, LDS bytes, neurons per workgroup when batched).  No counterpart in
+ * the reference. */
 spires_status spires_ext_kernel_plan(spires_reservoir *r, int *info);
 
 /* Step arithmetic. */

@@ -466,9 +466,10 @@ class Reservoir:
 
     def kernel_plan(self):
         """Exact-mode launch plan (spires_ext_kernel_plan) as a dict."""
-        info = (C.c_int * 6)()
+        info = (C.c_int * 7)()
         _check(lib().spires_ext_kernel_plan(self.h, info))
-        return dict(zip(("kernel", "nb", "workgroups", "row_pull", "weights_lds", "lds_bytes"), list(info)))
+        return dict(zip(("kernel", "nb", "workgroups", "row_pull", "weights_lds", "lds_bytes", "batch_nb"),
+                        list(info)))
 
     def set_kernel_path(self, path):
         """0 automatic, 1 one launch per micro-step, 2 persistent kernel only."""

@@ -179,7 +179,7 @@ class Evaluator:
             if t.res is None:
                 t.res = t.future.result()
                 t.future = None
-                t.nb = t.res.kernel_plan()["nb"]
+                t.nb = t.res.kernel_plan()["batch_nb"]
                 t.bps = t.res.bps
                 self.trials_built += 1
         self.wait_seconds += time.perf_counter() - t0

@@ -775,7 +775,10 @@ struct Plan {
     size_t lds = 0;
 };
 
-bool persist_plan(spires_dev* g, int ncu, Plan* pl) {
+// batch: the plan for a batched launch, which prefers 32 neurons per
+// workgroup (more reservoirs per launch); a reservoir launched alone prefers
+// 16 when the device has enough CUs (less work per workgroup, a shorter step).
+bool persist_plan(spires_dev* g, int ncu, Plan* pl, bool batch = false) {
     if (g->backend != BK_SPARSE) return false;
     if (!(g->ntype == NT_FLIF_GL || g->ntype == NT_LIF_DISCRETE || g->ntype == NT_LIF_BIO)) return false;
     if (!(g->stype == ST_SIMPLE || g->stype == ST_PSC_HET)) return false;   // PSC-hom runs as per-edge PSC
@@ -841,9 +844,10 @@ bool persist_plan(spires_dev* g, int ncu, Plan* pl) {
         // that keeps the row pull in LDS), then the single-wave chain with the
         // history in LDS, then in HBM; path 3 / 4 force the single-wave HBM / LDS variants.
         if (g->path == 0 || g->path == 2) {
-            for (int nb : {32, 16})
+            const int order[2] = {batch ? 32 : 16, batch ? 16 : 32};
+            for (int nb : order)
                 if (fill_tp(nb, true)) return true;
-            for (int nb : {32, 16})
+            for (int nb : order)
                 if (fill_tp(nb, false)) return true;
         }
         if (g->path != 3)
@@ -990,7 +994,7 @@ int batch_plan(spires_dev* const* gs, int count, int* nb_out, int* wgs_out) {
     int nb = 0, total = 0;
     for (int r = 0; r < count; ++r) {
         Plan pl;
-        if (!persist_plan(gs[r], ncu, &pl) || !pl.tp) {
+        if (!persist_plan(gs[r], ncu, &pl, true) || !pl.tp) {
             set_error("batched runs need FLIF-GL reservoirs that fit the product-wave kernel");
             return SPIRES_DEV_EINVAL;
         }
@@ -1017,7 +1021,7 @@ int launch_persist_batch(spires_dev* const* gs, int count, long long t0, long lo
     for (int r = 0; r < count; ++r) {
         spires_dev* g = gs[r];
         Plan pl;
-        persist_plan(g, ncu, &pl);
+        persist_plan(g, ncu, &pl, true);
         if ((rc = prepare_launch(g, pl))) return rc;
         SDEV_CK(hipStreamSynchronize(g->stream));
         table[r] = make_args(g, pl, g->din > 0 ? g->u_series : nullptr, t0, T, rec_V[r], rec_S[r]);
@@ -1055,15 +1059,17 @@ int launch_persist_batch(spires_dev* const* gs, int count, long long t0, long lo
 
 extern "C" int spires_dev_plan_info(spires_dev* g, int* info) {
     if (!g || !info) return SPIRES_DEV_EINVAL;
-    for (int k = 0; k < 6; ++k) info[k] = 0;
-    sdev::Plan pl;
-    if (g->mode != SPIRES_DEV_MODE_EXACT || g->path == 1 || !sdev::persist_plan(g, sdev::num_cus(), &pl)) return 0;
+    for (int k = 0; k < 7; ++k) info[k] = 0;
+    sdev::Plan pl, pb;
+    const int ncu = sdev::num_cus();
+    if (g->mode != SPIRES_DEV_MODE_EXACT || g->path == 1 || !sdev::persist_plan(g, ncu, &pl)) return 0;
     info[0] = pl.tp ? 2 : 1;
     info[1] = pl.nb;
     info[2] = pl.G;
     info[3] = pl.rpm;
     info[4] = pl.ne_lds > 0 ? 1 : 0;
     info[5] = (int)pl.lds;
+    info[6] = sdev::persist_plan(g, ncu, &pb, true) && pb.tp ? pb.nb : 0;
     return 0;
 }
 
@@ -1140,7 +1146,7 @@ extern "C" int spires_dev_timeblock_create(spires_dev* const* gs, int count, lon
     for (int r = 0; r < count; ++r) {
         spires_dev* g = gs[r];
         if (!g) return SPIRES_DEV_EINVAL;
-        if (g->mode != SPIRES_DEV_MODE_EXACT || g->path == 1 || !persist_plan(g, ncu, &plans[r]) || !plans[r].tp) {
+        if (g->mode != SPIRES_DEV_MODE_EXACT || g->path == 1 || !persist_plan(g, ncu, &plans[r], true) || !plans[r].tp) {
             set_error("time blocks need exact-mode FLIF-GL reservoirs that fit the product-wave kernel");
             return SPIRES_DEV_EINVAL;
         }

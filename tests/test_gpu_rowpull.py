@@ -15,9 +15,11 @@ from oracle import pyoracle as po
 pytestmark = pytest.mark.gpu
 
 CASES = [  # neurons, connectivity, history length, expected (row_pull, weights_lds, neurons per workgroup)
-    (2048, 0.005, 128, (0, 1, 32)),
-    (2048, 0.1, 128, (1, 1, 32)),
+    (2048, 0.005, 128, (0, 1, 16)),
+    (2048, 0.1, 128, (1, 1, 16)),
     (2048, 0.2, 900, (1, 0, 16)),
+    (5000, 0.004, 128, (0, 1, 32)),   # 16 per workgroup would need more workgroups than CUs
+    (5000, 0.04, 128, (1, 1, 32)),
     (4452, 0.05, 512, (0, 0, 32)),    # neither the edge list nor the bitmaps fit next to the history
 ]
 
