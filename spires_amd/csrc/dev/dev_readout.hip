@@ -45,11 +45,16 @@ static ReadoutCtx* readout_ctx() {
 }
 
 // Locks the current device's context, creates its handle on first use and
-// binds it to g's stream.
+// binds it to g's stream for the duration of the call.  On release the
+// stream is drained and the handle goes back to the null stream: reservoirs
+// (and their streams) come and go, and the shared handle must never keep a
+// stream that may be destroyed, nor let its workspace be reused on another
+// stream while work from this call is still in flight.
 struct ReadoutLock {
     ReadoutCtx* c = nullptr;
     std::unique_lock<std::mutex> lk;
     rocblas_handle h = nullptr;
+    hipStream_t bound = nullptr;
     explicit ReadoutLock(spires_dev* g) : c(readout_ctx()) {
         if (!c) return;
         lk = std::unique_lock<std::mutex>(c->mu);
@@ -58,6 +63,13 @@ struct ReadoutLock {
         rocblas_set_stream(c->h, g->stream);
         rocblas_set_pointer_mode(c->h, rocblas_pointer_mode_host);
         h = c->h;
+        bound = g->stream;
+    }
+    ~ReadoutLock() {
+        if (h) {
+            (void)hipStreamSynchronize(bound);
+            rocblas_set_stream(h, nullptr);
+        }
     }
     // Device scratch of at least `doubles` / `ints` entries (contents undefined).
     int reserve(spires_dev* g, size_t doubles, size_t ints) {
