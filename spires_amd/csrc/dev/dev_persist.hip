@@ -175,24 +175,51 @@ __device__ __forceinline__ double gl_seg_row(double h, const double* __restrict_
     return h;
 }
 
+// h += sum_{q<cnt} c[kb+q] * ring[(sb-q)*stride], ascending q, one multiply and
+// one add per lag (flif_gl.c:66-72), over a history ring in HBM.  The ring is
+// read in blocks of 16 lags with kHbmBlocks blocks in flight (the wave keeps
+// 16 * kHbmBlocks samples in registers), so the step waits on HBM bandwidth
+// rather than on one load latency per block; the final partial block is
+// loaded as a whole before its adds.
+constexpr int kHbmBlocks = 4;
 __device__ __forceinline__ double gl_seg_hbm(double h, const double* __restrict__ ring, const double* __restrict__ c,
                                              int kb, int sb, int cnt, size_t stride) {
     constexpr int B = 16;
-    int q = 0;
-    for (; q + B <= cnt; q += B) {
-        double v[B];
-        const double* rb = ring + (size_t)(sb - q) * stride;
+    const int nblk = cnt / B;
+    double v[kHbmBlocks][B];
+    auto load = [&](double* buf, int blk) {
+        const double* rb = ring + (size_t)(sb - blk * B) * stride;
 #pragma unroll
-        for (int j = 0; j < B; ++j) v[j] = rb[-(ptrdiff_t)(j * stride)];
+        for (int j = 0; j < B; ++j) buf[j] = rb[-(ptrdiff_t)(j * stride)];
+    };
 #pragma unroll
-        for (int j = 0; j < B; ++j) h = h + c[kb + q + j] * v[j];
+    for (int u = 0; u < kHbmBlocks; ++u)
+        if (u < nblk) load(v[u], u);
+    for (int b0 = 0; b0 < nblk; b0 += kHbmBlocks) {
+#pragma unroll
+        for (int u = 0; u < kHbmBlocks; ++u) {
+            const int blk = b0 + u;
+            if (blk < nblk) {
+#pragma unroll
+                for (int j = 0; j < B; ++j) h = h + c[kb + blk * B + j] * v[u][j];
+                if (blk + kHbmBlocks < nblk) load(v[u], blk + kHbmBlocks);
+            }
+        }
     }
-    for (; q < cnt; ++q) h = h + c[kb + q] * ring[(size_t)(sb - q) * stride];
+    const int q = nblk * B, rem = cnt - q;
+    if (rem > 0) {
+        double t[B];
+#pragma unroll
+        for (int j = 0; j < B; ++j) t[j] = j < rem ? ring[(size_t)(sb - q - j) * stride] : 0.0;
+#pragma unroll
+        for (int j = 0; j < B; ++j)
+            if (j < rem) h = h + c[kb + q + j] * t[j];
+    }
     return h;
 }
 
 template <int NT, int NB, bool RL>
-__global__ void __launch_bounds__(kPersistThreads) k_persist(PersistArgs a) {
+__global__ void __launch_bounds__(kPersistThreads) __attribute__((amdgpu_waves_per_eu(1, 1))) k_persist(PersistArgs a) {
     extern __shared__ __align__(16) double lds[];   // 16-byte aligned: b128 ring-pair reads
     constexpr bool GL = (NT == NT_FLIF_GL);
     // wave index and abort flag are read through readfirstlane so the compiler
