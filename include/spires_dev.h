@@ -162,6 +162,20 @@ void *spires_dev_stream(spires_dev *g);
 size_t spires_dev_steps(const spires_dev *g);
 const char *spires_dev_last_error(void);
 
+/* Reservoir construction on the current device.  Erdos-Renyi W (edge i->j,
+ * i != j, with probability p; magnitude U(0,1], positive with probability ei;
+ * SplitMix64 of (seed, i, j), so one seed gives one matrix on any device)
+ * generated in CSR with ascending columns, its spectral radius estimated by
+ * the reference power iteration and W scaled by rho / lambda when
+ * lambda > 1e-9.  n <= 19968.  Outputs are malloc'd (the caller free()s
+ * them); *lambda gets the unscaled estimate. */
+int  spires_dev_random_er(int n, double p, double ei, unsigned long long seed, double rho, int64_t **row_ptr,
+                          int32_t **col_idx, double **values, size_t *nnz, double *lambda);
+/* The reference power iteration (sparse.c:93-127) on the device for a host
+ * CSR matrix; *lambda is bit-identical to the CPU loop's, *iters its count. */
+int  spires_dev_spectral_radius(int n, const int64_t *row_ptr, const int32_t *col_idx,
+                                const double *values, double *lambda, int *iters);
+
 #ifdef __cplusplus
 }
 #endif

@@ -42,6 +42,35 @@ spires_status spires_ext_build_weights(const spires_reservoir_config *cfg, size_
 double spires_ext_spectral_radius(size_t n, const int64_t *row_ptr, const int32_t *col_idx,
                                   const double *values);
 
+/* Reservoir construction on the GPU (replaces the host rand() draws and the
+ * dense N x N scratch of init_weights, reservoir.c:345-545, and the host
+ * power iteration + rescale, sparse.c:89-127 / reservoir.c:548-557, for the
+ * random topology).  Edge i->j (i != j) with probability p, magnitude
+ * U(0,1], positive with probability ei -- generate_weight's distribution --
+ * hashed from (seed, i, j) with SplitMix64: a different random stream from
+ * the reference's rand(), so the matrix agrees with the reference's in
+ * distribution, not entry by entry.  W is scaled by rho / lambda
+ * (lambda > 1e-9) with lambda from the reference power iteration, run on the
+ * device; *lambda (may be NULL) gets the unscaled estimate.  n <= 19968.
+ * Outputs are malloc'd; the caller free()s them. */
+spires_status spires_ext_random_er(size_t n, double p, double ei, uint64_t seed, double rho,
+                                   size_t *nnz, int64_t **row_ptr, int32_t **col_idx,
+                                   double **values, double *lambda);
+/* spires_reservoir_create (spires.h) for SPIRES_CONN_RANDOM with W built by
+ * spires_ext_random_er(N, connectivity, ei_ratio, seed, spectral_radius) and
+ * W_in / W_out from SplitMix64 streams of the same seed (generate_weight's
+ * distribution; ei_ratio for W_in, 0.5 for W_out).  Synapse time constants
+ * still come from rand(), as in spires_reservoir_create.  Other topologies
+ * return SPIRES_ERR_INVALID_ARG. */
+spires_status spires_ext_create_device(const spires_reservoir_config *cfg, uint64_t seed,
+                                       spires_reservoir **out);
+/* spires_ext_spectral_radius on the device (n <= 19968): the same operations
+ * in the same order, so *lambda is bit-identical; *iters (may be NULL) gets
+ * the number of sparse products run. */
+spires_status spires_ext_spectral_radius_device(size_t n, const int64_t *row_ptr,
+                                                const int32_t *col_idx, const double *values,
+                                                double *lambda, int *iters);
+
 /* Run T macro steps over u [T x Din] and copy back any of: states [T x N]
  * (V after each step), raster [T x N] (u8 spikes), outputs [T x Dout]. */
 spires_status spires_ext_run_record(spires_reservoir *r, const double *u, size_t T,

@@ -96,6 +96,7 @@ SIGNATURES = {
     "spires_num_outputs": (_SZ, [_P]),
     "spires_optimize": (C.c_int, [C.POINTER(Config), C.POINTER(OptBudget), C.c_int, C.POINTER(OptScore),
                                   C.POINTER(OptResult), _D, _D, _SZ]),
+    "spires_ext_create_device": (C.c_int, [C.POINTER(Config), C.c_uint64, C.POINTER(_P)]),
     "spires_ext_create_csr": (C.c_int, [C.POINTER(Config), _SZ, _I64, _I32, _D, _D, _D, C.POINTER(_P)]),
     "spires_ext_run_record": (C.c_int, [_P, _D, _SZ, _D, _U8, _D]),
     "spires_ext_upload_inputs": (C.c_int, [_P, _D, _SZ]),
@@ -133,6 +134,11 @@ SIGNATURES = {
     "spires_ext_mlp_gradient": (C.c_int, [_P, _F, _F, C.c_int, _F, _F]),
     "spires_ext_mlp_train": (C.c_int, [_P, _F, _F, C.c_int, C.c_int, C.c_float, _F]),
     "spires_ext_mlp_last_error": (C.c_char_p, []),
+    "spires_ext_random_er": (C.c_int, [_SZ, C.c_double, C.c_double, C.c_uint64, C.c_double, C.POINTER(_SZ),
+                                       C.POINTER(_I64),
+                                       C.POINTER(_I32), C.POINTER(_D), C.POINTER(C.c_double)]),
+    "spires_ext_spectral_radius_device": (C.c_int, [_SZ, _I64, _I32, _D, C.POINTER(C.c_double),
+                                                    C.POINTER(C.c_int)]),
     "spires_ext_build_weights": (C.c_int, [C.POINTER(Config), C.POINTER(_SZ), C.POINTER(_I64), C.POINTER(_I32),
                                            C.POINTER(_D), C.POINTER(_D), C.POINTER(_D)]),
 }
@@ -212,6 +218,38 @@ def spectral_radius(rp, ci, val):
     val = _f64(val)
     return lib().spires_ext_spectral_radius(len(rp) - 1, rp.ctypes.data_as(_I64), ci.ctypes.data_as(_I32),
                                             _dp(val))
+
+
+def spectral_radius_device(rp, ci, val):
+    """The reference power iteration on the GPU: (lambda, sparse products run);
+    lambda is bit-identical to spectral_radius()."""
+    rp = np.ascontiguousarray(rp, np.int64)
+    ci = np.ascontiguousarray(ci, np.int32)
+    val = np.ascontiguousarray(val, np.float64)
+    lam, it = C.c_double(), C.c_int()
+    _check(lib().spires_ext_spectral_radius_device(len(rp) - 1, rp.ctypes.data_as(_I64), ci.ctypes.data_as(_I32),
+                                                   val.ctypes.data_as(_D), C.byref(lam), C.byref(it)))
+    return lam.value, it.value
+
+
+def random_er(n, p, seed, rho, ei=0.8):
+    """Erdos-Renyi recurrent weights built on the GPU: (row_ptr, col_idx,
+    values, lambda) with values scaled to spectral radius rho (lambda is the
+    power-iteration estimate before scaling)."""
+    nnz, lam = _SZ(), C.c_double()
+    rp, ci, val = _I64(), _I32(), _D()
+    _check(lib().spires_ext_random_er(n, float(p), float(ei), C.c_uint64(seed), float(rho), C.byref(nnz), C.byref(rp),
+                                      C.byref(ci), C.byref(val), C.byref(lam)))
+    k = nnz.value
+    try:
+        out = (np.ctypeslib.as_array(rp, shape=(n + 1,)).copy(),
+               np.ctypeslib.as_array(ci, shape=(k,)).copy() if k else np.zeros(0, np.int32),
+               np.ctypeslib.as_array(val, shape=(k,)).copy() if k else np.zeros(0, np.float64),
+               lam.value)
+    finally:
+        for q in (rp, ci, val):
+            _libc.free(C.cast(q, _P))
+    return out
 
 
 def build_weights(cfg_tuple):
@@ -361,13 +399,18 @@ class Surrogate:
 class Reservoir:
     """Object view of a spires_reservoir handle (spires.h names)."""
 
-    def __init__(self, cfg_tuple, csr=None, W_in=None, W_out=None):
+    def __init__(self, cfg_tuple, csr=None, W_in=None, W_out=None, device_seed=None):
+        """csr=(row_ptr, col_idx, values) with W_in/W_out: given weights;
+        device_seed: random topology built on the GPU from that seed
+        (spires_ext_create_device); neither: spires_reservoir_create."""
         self._keep = cfg_tuple
         cfg = cfg_tuple[0]
         self.N, self.Din, self.Dout = cfg.num_neurons, cfg.num_inputs, cfg.num_outputs
         h = _P()
         L = lib()
-        if csr is None:
+        if device_seed is not None:
+            _check(L.spires_ext_create_device(C.byref(cfg), C.c_uint64(device_seed), C.byref(h)))
+        elif csr is None:
             _check(L.spires_reservoir_create(C.byref(cfg), C.byref(h)))
         else:
             rp, ci, val = csr

@@ -764,6 +764,61 @@ double spires_ext_spectral_radius(size_t n, const int64_t* row_ptr, const int32_
     return csr_spectral_radius(W);
 }
 
+spires_status spires_ext_random_er(size_t n, double p, double ei, uint64_t seed, double rho, size_t* nnz,
+                                   int64_t** row_ptr, int32_t** col_idx, double** values, double* lambda) {
+    if (n == 0 || n > (size_t)INT32_MAX || !nnz || !row_ptr || !col_idx || !values) return SPIRES_ERR_INVALID_ARG;
+    return dev_status(spires_dev_random_er((int)n, p, ei, (unsigned long long)seed, rho, row_ptr, col_idx, values,
+                                           nnz, lambda));
+}
+
+// init_reservoir (reservoir.c:68-78) with the recurrent matrix built and
+// rescaled on the device.  W_in and W_out keep generate_weight's distribution
+// (magnitude U(0,1], sign by ei_ratio; 0.5 for W_out, reservoir.c:559-571),
+// drawn from SplitMix64 streams of the same seed.
+spires_status spires_ext_create_device(const spires_reservoir_config* cfg, uint64_t seed, spires_reservoir** out_r) {
+    if (!cfg || !out_r) return SPIRES_ERR_INVALID_ARG;
+    spires_status s = validate_cfg(cfg);
+    if (s != SPIRES_OK) return s;
+    if (cfg->connectivity_type != SPIRES_CONN_RANDOM) {
+        set_error("device construction builds the random (Erdos-Renyi) topology only");
+        return SPIRES_ERR_INVALID_ARG;
+    }
+    const size_t n = cfg->num_neurons;
+    size_t nnz = 0;
+    int64_t* rp = nullptr;
+    int32_t* ci = nullptr;
+    double* val = nullptr;
+    s = spires_ext_random_er(n, cfg->connectivity, cfg->ei_ratio, seed, cfg->spectral_radius, &nnz, &rp, &ci, &val,
+                             nullptr);
+    if (s != SPIRES_OK) return s;
+    std::vector<double> W_in(n * cfg->num_inputs), W_out(n * cfg->num_outputs);
+    uint64_t z = seed ^ 0xD1B54A32D192ED03ULL;
+    auto unit = [&]() {   // SplitMix64, 53 bits in [0, 1)
+        z += 0x9E3779B97F4A7C15ULL;
+        uint64_t h = z;
+        h = (h ^ (h >> 30)) * 0xBF58476D1CE4E5B9ULL;
+        h = (h ^ (h >> 27)) * 0x94D049BB133111EBULL;
+        return (double)((h ^ (h >> 31)) >> 11) * (1.0 / 9007199254740992.0);
+    };
+    auto weight = [&](double ei) {
+        const double mag = 1.0 - unit();   // (0, 1]
+        return unit() < ei ? mag : -mag;
+    };
+    for (double& w : W_in) w = weight(cfg->ei_ratio);
+    for (double& w : W_out) w = weight(0.5);
+    s = spires_ext_create_csr(cfg, nnz, rp, ci, val, W_in.data(), W_out.data(), out_r);
+    std::free(rp);
+    std::free(ci);
+    std::free(val);
+    return s;
+}
+
+spires_status spires_ext_spectral_radius_device(size_t n, const int64_t* row_ptr, const int32_t* col_idx,
+                                                const double* values, double* lambda, int* iters) {
+    if (n == 0 || n > (size_t)INT32_MAX || !row_ptr || !lambda) return SPIRES_ERR_INVALID_ARG;
+    return dev_status(spires_dev_spectral_radius((int)n, row_ptr, col_idx, values, lambda, iters));
+}
+
 spires_status spires_ext_build_weights(const spires_reservoir_config* cfg, size_t* nnz_out, int64_t** row_ptr,
                                        int32_t** col_idx, double** values, double** W_in, double** W_out) {
     if (!cfg || !nnz_out || !row_ptr || !col_idx || !values || !W_in || !W_out) return SPIRES_ERR_INVALID_ARG;
