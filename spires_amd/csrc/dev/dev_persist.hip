@@ -68,6 +68,20 @@ struct PersistArgs {
 __device__ __forceinline__ void add_f64_ordered(double& h, double p) {
     asm volatile("v_add_f64 %0, %0, %1" : "+v"(h) : "v"(p));
 }
+// Sixteen dependent adds h += b[0].x, b[0].y, ..., b[7].y in one statement:
+// back-to-back dependent v_add_f64 need no wait state, but the compiler puts
+// one between consecutive inline-asm statements, which costs about as much as
+// the add itself on a latency-bound chain.
+__device__ __forceinline__ void add16_f64_ordered(double& h, const double2* b) {
+    asm volatile(
+        "v_add_f64 %0, %0, %1\n\tv_add_f64 %0, %0, %2\n\tv_add_f64 %0, %0, %3\n\tv_add_f64 %0, %0, %4\n\t"
+        "v_add_f64 %0, %0, %5\n\tv_add_f64 %0, %0, %6\n\tv_add_f64 %0, %0, %7\n\tv_add_f64 %0, %0, %8\n\t"
+        "v_add_f64 %0, %0, %9\n\tv_add_f64 %0, %0, %10\n\tv_add_f64 %0, %0, %11\n\tv_add_f64 %0, %0, %12\n\t"
+        "v_add_f64 %0, %0, %13\n\tv_add_f64 %0, %0, %14\n\tv_add_f64 %0, %0, %15\n\tv_add_f64 %0, %0, %16"
+        : "+v"(h)
+        : "v"(b[0].x), "v"(b[0].y), "v"(b[1].x), "v"(b[1].y), "v"(b[2].x), "v"(b[2].y), "v"(b[3].x), "v"(b[3].y),
+          "v"(b[4].x), "v"(b[4].y), "v"(b[5].x), "v"(b[5].y), "v"(b[6].x), "v"(b[6].y), "v"(b[7].x), "v"(b[7].y));
+}
 __device__ __forceinline__ void mul_f64_ordered(double& r, double a, double b) {
     asm volatile("v_mul_f64 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
 }
@@ -445,6 +459,9 @@ __device__ __forceinline__ void lds_counter_store(unsigned* f, unsigned v) {
 // and 2*PS = 4 (mod 32) dwords keeps the rows' 16-byte reads conflict-free.
 __host__ __device__ __forceinline__ int tp_row_stride(int L) { return 32 * ((L - 2 + 31) / 32) + 2; }
 
+template <int R_>
+struct RoleTag { static constexpr int value = R_; };
+
 template <int NB>
 __device__ __forceinline__ void tp_body(const PersistArgs& a, const int g) {
     constexpr int C = kTpChunk;
@@ -580,197 +597,206 @@ __device__ __forceinline__ void tp_body(const PersistArgs& a, const int g) {
     const double2* prow = reinterpret_cast<const double2*>(P + (size_t)r * PS);   // pair j: lags 2j, 2j+1
     long long st = st0;
     unsigned m = 0;                                    // micro-step index within the launch
-    for (long long t = 0; t < a.T; ++t) {
-        const long long tt = t0 + t;
-        const int tb = (int)(t & 1);
-        for (int mi = 0; mi < a.micro; ++mi, ++st, ++m) {
-            const int sb = (int)(st & 1);
-            const int nk = lim >= 2 ? lim - 1 : 0;     // lags 2..lim of this step
-            const int nch = (nk + C - 1) / C;
-            if (wave == 0) {
-                // ---------------- chain: H = c_1 V_{st-1} + sum_k P[k]
-                const double Vprev = stage[(int)((st - 1) & 1) * NB + r];
-                double h = 0.0;
-                if (lim >= 1) h = h + c1 * Vprev;
-                double2 bA[C / 2], bB[C / 2];
-                auto ld = [&](double2* buf, int c) {
+    // One loop per wave role: with the roles as branches of a single loop, copies
+    // of the producers' in-flight load registers sit on every path and make the
+    // chain wave wait for its own global stores (the store counter is shared).
+    auto steps = [&](auto role) __attribute__((always_inline)) {
+        constexpr int R = decltype(role)::value;
+        for (long long t = 0; t < a.T; ++t) {
+            const long long tt = t0 + t;
+            const int tb = (int)(t & 1);
+            for (int mi = 0; mi < a.micro; ++mi, ++st, ++m) {
+                const int sb = (int)(st & 1);
+                const int nk = lim >= 2 ? lim - 1 : 0;     // lags 2..lim of this step
+                const int nch = (nk + C - 1) / C;
+                if constexpr (R == 0) {
+                    // ---------------- chain: H = c_1 V_{st-1} + sum_k P[k]
+                    const double Vprev = stage[(int)((st - 1) & 1) * NB + r];
+                    double h = 0.0;
+                    if (lim >= 1) h = h + c1 * Vprev;
+                    double2 bA[C / 2], bB[C / 2];
+                    auto ld = [&](double2* buf, int c) {
 #pragma unroll
-                    for (int j = 0; j < C / 2; ++j) buf[j] = prow[1 + (C / 2) * c + j];
-                };
-                auto add = [&](const double2* buf, int cnt) {
-                    if (cnt == C) {
+                        for (int j = 0; j < C / 2; ++j) buf[j] = prow[1 + (C / 2) * c + j];
+                    };
+                    auto add = [&](const double2* buf, int cnt) {
+                        if (cnt == C) {
+                            add16_f64_ordered(h, buf);
+                            add16_f64_ordered(h, buf + 8);
+                        } else {
 #pragma unroll
-                        for (int j = 0; j < C / 2; ++j) { add_f64_ordered(h, buf[j].x); add_f64_ordered(h, buf[j].y); }
-                    } else {
-#pragma unroll
-                        for (int j = 0; j < C / 2; ++j) {
-                            if (2 * j < cnt) add_f64_ordered(h, buf[j].x);
-                            if (2 * j + 1 < cnt) add_f64_ordered(h, buf[j].y);
+                            for (int j = 0; j < C / 2; ++j) {
+                                if (2 * j < cnt) add_f64_ordered(h, buf[j].x);
+                                if (2 * j + 1 < cnt) add_f64_ordered(h, buf[j].y);
+                            }
+                        }
+                    };
+                    if (lane < NB) {   // lanes past NB would only repeat the rows of lanes 0..NB-1
+                        if (nch > 0) ld(bA, 0);
+                        int c = 0;
+                        for (; c + 2 <= nch; c += 2) {
+                            ld(bB, c + 1);
+                            add(bA, C);
+                            lds_counter_store(&cons_s, m * 64u + (unsigned)c + 1u);
+                            if (c + 2 < nch) ld(bA, c + 2);
+                            add(bB, (c + 1 == nch - 1) ? nk - (c + 1) * C : C);
+                            lds_counter_store(&cons_s, m * 64u + (unsigned)c + 2u);
+                        }
+                        if (c < nch) {
+                            add(bA, nk - c * C);
+                            lds_counter_store(&cons_s, m * 64u + (unsigned)c + 1u);
                         }
                     }
-                };
-                if (lane < NB) {   // lanes past NB would only repeat the rows of lanes 0..NB-1
-                    if (nch > 0) ld(bA, 0);
-                    int c = 0;
-                    for (; c + 2 <= nch; c += 2) {
-                        ld(bB, c + 1);
-                        add(bA, C);
-                        lds_counter_store(&cons_s, m * 64u + (unsigned)c + 1u);
-                        if (c + 2 < nch) ld(bA, c + 2);
-                        add(bB, (c + 1 == nch - 1) ? nk - (c + 1) * C : C);
-                        lds_counter_store(&cons_s, m * 64u + (unsigned)c + 2u);
+                    const double leak = -(Vprev - p[2]) / p[3];   // before the barrier: the wait hides it
+                    __syncthreads();   // rec / ext of this step and the products of the next are ready
+                    if (__builtin_amdgcn_readfirstlane(abort_s)) break;
+                    if (own) {                                  // flif_gl.c:50-100
+                        const double x = ext_s[tb * NB + lane] + rec_s[sb * NB + lane];
+                        spk = false;
+                        if (tp > p[6]) {
+                            const double rhs = leak + x + p[5];
+                            V = p[4] * rhs - h;
+                            if (V >= p[0]) { V = p[1]; spk = true; tp = 0.0; }
+                        } else {
+                            V = p[1];
+                        }
                     }
-                    if (c < nch) {
-                        add(bA, nk - c * C);
-                        lds_counter_store(&cons_s, m * 64u + (unsigned)c + 1u);
+                    publish_spikes(a.xch, nw, g, wpg, st, lane, __ballot(own && spk));
+                    if (own) {
+                        a.ring[(size_t)head * n + i] = V;
+                        stage[sb * NB + lane] = V;
+                        tp = tp + a.dt;
+                        if (mi == a.micro - 1) {
+                            if (rec_V) rec_V[(size_t)t * n + i] = V;
+                            if (rec_S) rec_S[(size_t)t * n + i] = spk ? 1 : 0;
+                        }
                     }
-                }
-                const double leak = -(Vprev - p[2]) / p[3];   // before the barrier: the wait hides it
-                __syncthreads();   // rec / ext of this step and the products of the next are ready
-                if (__builtin_amdgcn_readfirstlane(abort_s)) break;
-                if (own) {                                  // flif_gl.c:50-100
-                    const double x = ext_s[tb * NB + lane] + rec_s[sb * NB + lane];
-                    spk = false;
-                    if (tp > p[6]) {
-                        const double rhs = leak + x + p[5];
-                        V = p[4] * rhs - h;
-                        if (V >= p[0]) { V = p[1]; spk = true; tp = 0.0; }
-                    } else {
-                        V = p[1];
-                    }
-                }
-                publish_spikes(a.xch, nw, g, wpg, st, lane, __ballot(own && spk));
-                if (own) {
-                    a.ring[(size_t)head * n + i] = V;
-                    stage[sb * NB + lane] = V;
-                    tp = tp + a.dt;
-                    if (mi == a.micro - 1) {
-                        if (rec_V) rec_V[(size_t)t * n + i] = V;
-                        if (rec_S) rec_S[(size_t)t * n + i] = spk ? 1 : 0;
-                    }
-                }
-                lds_counter_store(&cons_s, (m + 1u) * 64u);   // tail done: V_st staged
-            } else if (wave == 4) {
-                // ---------------- exchange, row pull, external input
-                double uin[kMaxDinReg];
-                if (mi == 0 && win_reg) {
+                    lds_counter_store(&cons_s, (m + 1u) * 64u);   // tail done: V_st staged
+                } else if constexpr (R == 4) {
+                    // ---------------- exchange, row pull, external input
+                    double uin[kMaxDinReg];
+                    if (mi == 0 && win_reg) {
 #pragma unroll
-                    for (int j = 0; j < kMaxDinReg; ++j) uin[j] = (j < a.din) ? a.u[tt * a.din + j] : 0.0;
-                }
-                const bool aborted = !gather_spikes(a.xch, nw, NB, wpg, n, st, st0, S_in, bits_s,
-                                                    a.abort_flag, &abort_s, lane);
-                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-                __builtin_amdgcn_wave_barrier();
-                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-                if (bitmap && !aborted) {
-                    if (NB == 16) {   // two 16-neuron exchange words per column-order word
-                        for (int w = lane; w < nwp; w += 64)
-                            pk_s[w] = bits_s[2 * w] | (2 * w + 1 < nw ? bits_s[2 * w + 1] << 16 : 0u);
+                        for (int j = 0; j < kMaxDinReg; ++j) uin[j] = (j < a.din) ? a.u[tt * a.din + j] : 0.0;
+                    }
+                    const bool aborted = !gather_spikes(a.xch, nw, NB, wpg, n, st, st0, S_in, bits_s,
+                                                        a.abort_flag, &abort_s, lane);
+                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                    __builtin_amdgcn_wave_barrier();
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                    if (bitmap && !aborted) {
+                        if (NB == 16) {   // two 16-neuron exchange words per column-order word
+                            for (int w = lane; w < nwp; w += 64)
+                                pk_s[w] = bits_s[2 * w] | (2 * w + 1 < nw ? bits_s[2 * w + 1] << 16 : 0u);
+                            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                            __builtin_amdgcn_wave_barrier();
+                            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                        }
+                        bitmap_scan(bm_s, pk_s, list_s, cnt_s, pb_s, nwp, a.cw, NB, (lane & (NB - 1)) < nloc, lane);
                         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
                         __builtin_amdgcn_wave_barrier();
                         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
                     }
-                    bitmap_scan(bm_s, pk_s, list_s, cnt_s, pb_s, nwp, a.cw, NB, (lane & (NB - 1)) < nloc, lane);
-                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-                    __builtin_amdgcn_wave_barrier();
-                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-                }
-                if (!aborted && lane < nloc) {
-                    double rec;
-                    if (bitmap)
-                        rec = csr_lds ? bitmap_sum<kBitmapBatch / 2>(list_s, cnt_s, pb_s, eval_s, NB, lane)
-                                      : bitmap_sum<kBitmapBatch / 2>(list_s, cnt_s, pb_s, a.val + E0, NB, lane);
-                    else
-                        rec = row_pull(a.stype, csr_lds, rs, re, E0, eidx_s, eval_s, a.col, a.val, bits_s, NB, wpg,
-                                       a.edge_decay, a.edge_tr);
-                    rec_s[sb * NB + lane] = rec;
-                    if (mi == 0) {
-                        double su = 0.0;
-                        if (win_reg) {
+                    if (!aborted && lane < nloc) {
+                        double rec;
+                        if (bitmap)
+                            rec = csr_lds ? bitmap_sum<kBitmapBatch / 2>(list_s, cnt_s, pb_s, eval_s, NB, lane)
+                                          : bitmap_sum<kBitmapBatch / 2>(list_s, cnt_s, pb_s, a.val + E0, NB, lane);
+                        else
+                            rec = row_pull(a.stype, csr_lds, rs, re, E0, eidx_s, eval_s, a.col, a.val, bits_s, NB, wpg,
+                                           a.edge_decay, a.edge_tr);
+                        rec_s[sb * NB + lane] = rec;
+                        if (mi == 0) {
+                            double su = 0.0;
+                            if (win_reg) {
 #pragma unroll
-                            for (int j = 0; j < kMaxDinReg; ++j)
-                                if (j < a.din) su = su + win[j] * uin[j];
-                        } else {
-                            for (int j = 0; j < a.din; ++j) su = su + a.W_in[(size_t)i * a.din + j] * a.u[tt * a.din + j];
+                                for (int j = 0; j < kMaxDinReg; ++j)
+                                    if (j < a.din) su = su + win[j] * uin[j];
+                            } else {
+                                for (int j = 0; j < a.din; ++j) su = su + a.W_in[(size_t)i * a.din + j] * a.u[tt * a.din + j];
+                            }
+                            ext_s[tb * NB + lane] = a.gain * su;
                         }
-                        ext_s[tb * NB + lane] = a.gain * su;
                     }
-                }
-                __syncthreads();
-                if (__builtin_amdgcn_readfirstlane(abort_s)) break;
-            } else {
-                // ---------------- products of step st+1, behind the chain of step st
-                const int headn = head + 1 == L ? 0 : head + 1;
-                const int limn = lim < L - 1 ? lim + 1 : L - 1;
-                const int nkn = limn >= 2 ? limn - 1 : 0;
-                const int nchn = (nkn + C - 1) / C;
-                // V_{st+1-k}, k >= 3, is final since the last barrier: loads run three chunks
-                // ahead of the stores.  The slot of lag k = kbase + off is
-                // (headn - kbase - off) mod L with off < 2L; lag 2 (replaced by the staged
-                // value) and lags past lim are read but discarded.
-                const int sbase = headn - (2 + part * PER);
-                const double* ring_r = a.ring + ir;
-                auto ldr = [&](double* v, int c) {
+                    __syncthreads();
+                    if (__builtin_amdgcn_readfirstlane(abort_s)) break;
+                } else {
+                    // ---------------- products of step st+1, behind the chain of step st
+                    const int headn = head + 1 == L ? 0 : head + 1;
+                    const int limn = lim < L - 1 ? lim + 1 : L - 1;
+                    const int nkn = limn >= 2 ? limn - 1 : 0;
+                    const int nchn = (nkn + C - 1) / C;
+                    // V_{st+1-k}, k >= 3, is final since the last barrier: loads run three chunks
+                    // ahead of the stores.  The slot of lag k = kbase + off is
+                    // (headn - kbase - off) mod L with off < 2L; lag 2 (replaced by the staged
+                    // value) and lags past lim are read but discarded.
+                    const int sbase = headn - (2 + part * PER);
+                    const double* ring_r = a.ring + ir;
+                    auto ldr = [&](double* v, int c) {
 #pragma unroll
-                    for (int j = 0; j < PER; ++j) {
-                        int s = sbase - (c * C + j);
-                        s += s < 0 ? L : 0;
-                        s += s < 0 ? L : 0;
-                        v[j] = ring_r[(size_t)s * n];
-                    }
-                };
-                bool stalled = false;
-                auto put = [&](const double* v, int c) {
-                    double2 pr[PER / 2];
-#pragma unroll
-                    for (int j = 0; j < PER / 2; ++j) {
-                        const int k = 2 + c * C + part * PER + 2 * j;
-                        pr[j] = make_double2(coef_s[k] * v[2 * j], coef_s[k + 1] * v[2 * j + 1]);
-                    }
-                    // the chain of step st is past chunk c (or does not read it) and its tail
-                    // of step st-1 has staged V_{st-1}
-                    const unsigned target = m * 64u + (unsigned)(c + 1 < nch ? c + 1 : nch);
-                    long long spins = 0;
-                    while ((int)(lds_counter_load(&cons_s) - target) < 0 && !stalled) {
-                        if (++spins > kTpSpinLimit) {
-                            stalled = true;
-                            __hip_atomic_store(&abort_s, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-                            __hip_atomic_store(a.abort_flag, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        for (int j = 0; j < PER; ++j) {
+                            int s = sbase - (c * C + j);
+                            s += s < 0 ? L : 0;
+                            s += s < 0 ? L : 0;
+                            v[j] = ring_r[(size_t)s * n];
                         }
-                        __builtin_amdgcn_s_sleep(1);
-                    }
-                    if (c == 0 && part == 0)
-                        pr[0].x = coef_s[2] * stage[(int)((st - 1) & 1) * NB + r];   // lag 2: V_{st-1}
-                    double2* dst = reinterpret_cast<double2*>(P + (size_t)r * PS + 2 + c * C + part * PER);
+                    };
+                    bool stalled = false;
+                    auto put = [&](const double* v, int c) {
+                        double2 pr[PER / 2];
 #pragma unroll
-                    for (int j = 0; j < PER / 2; ++j) dst[j] = pr[j];
-                };
-                double vA[PER], vB[PER], vC[PER], vD[PER];
-                if (nchn > 0) ldr(vA, 0);
-                if (nchn > 1) ldr(vB, 1);
-                if (nchn > 2) ldr(vC, 2);
-                int c = 0;
-                for (; c + 4 <= nchn; c += 4) {
-                    ldr(vD, c + 3);
-                    put(vA, c);
-                    if (c + 4 < nchn) ldr(vA, c + 4);
-                    put(vB, c + 1);
-                    if (c + 5 < nchn) ldr(vB, c + 5);
-                    put(vC, c + 2);
-                    if (c + 6 < nchn) ldr(vC, c + 6);
-                    put(vD, c + 3);
+                        for (int j = 0; j < PER / 2; ++j) {
+                            const int k = 2 + c * C + part * PER + 2 * j;
+                            pr[j] = make_double2(coef_s[k] * v[2 * j], coef_s[k + 1] * v[2 * j + 1]);
+                        }
+                        // the chain of step st is past chunk c (or does not read it) and its tail
+                        // of step st-1 has staged V_{st-1}
+                        const unsigned target = m * 64u + (unsigned)(c + 1 < nch ? c + 1 : nch);
+                        long long spins = 0;
+                        while ((int)(lds_counter_load(&cons_s) - target) < 0 && !stalled) {
+                            if (++spins > kTpSpinLimit) {
+                                stalled = true;
+                                __hip_atomic_store(&abort_s, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                                __hip_atomic_store(a.abort_flag, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                            }
+                            __builtin_amdgcn_s_sleep(1);
+                        }
+                        if (c == 0 && part == 0)
+                            pr[0].x = coef_s[2] * stage[(int)((st - 1) & 1) * NB + r];   // lag 2: V_{st-1}
+                        double2* dst = reinterpret_cast<double2*>(P + (size_t)r * PS + 2 + c * C + part * PER);
+#pragma unroll
+                        for (int j = 0; j < PER / 2; ++j) dst[j] = pr[j];
+                    };
+                    double vA[PER], vB[PER], vC[PER], vD[PER];
+                    if (nchn > 0) ldr(vA, 0);
+                    if (nchn > 1) ldr(vB, 1);
+                    if (nchn > 2) ldr(vC, 2);
+                    int c = 0;
+                    for (; c + 4 <= nchn; c += 4) {
+                        ldr(vD, c + 3);
+                        put(vA, c);
+                        if (c + 4 < nchn) ldr(vA, c + 4);
+                        put(vB, c + 1);
+                        if (c + 5 < nchn) ldr(vB, c + 5);
+                        put(vC, c + 2);
+                        if (c + 6 < nchn) ldr(vC, c + 6);
+                        put(vD, c + 3);
+                    }
+                    if (c < nchn) put(vA, c);
+                    if (c + 1 < nchn) put(vB, c + 1);
+                    if (c + 2 < nchn) put(vC, c + 2);
+                    __syncthreads();
+                    if (__builtin_amdgcn_readfirstlane(abort_s)) break;
                 }
-                if (c < nchn) put(vA, c);
-                if (c + 1 < nchn) put(vB, c + 1);
-                if (c + 2 < nchn) put(vC, c + 2);
-                __syncthreads();
-                if (__builtin_amdgcn_readfirstlane(abort_s)) break;
+                head = head + 1 == L ? 0 : head + 1;
+                if (lim < L - 1) ++lim;
             }
-            head = head + 1 == L ? 0 : head + 1;
-            if (lim < L - 1) ++lim;
+            if (__builtin_amdgcn_readfirstlane(abort_s)) break;
         }
-        if (__builtin_amdgcn_readfirstlane(abort_s)) break;
-    }
+    };
+    if (wave == 0) steps(RoleTag<0>{});
+    else if (wave == 4) steps(RoleTag<4>{});
+    else steps(RoleTag<1>{});
     if (own) {
         a.V[i] = V;
         S_out[i] = spk ? 1 : 0;
