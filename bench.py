@@ -18,12 +18,12 @@ With --copy-calibration a plain device-to-device copy of 256 MiB runs first:
 its FETCH_SIZE / WRITE_SIZE under rocprofv3 --pmc calibrate the counters that
 give the step kernel's HBM traffic.
 
-With --agile it runs the config-4 hyperparameter search instead: the plain
+With --agile it runs the AGILE hyperparameter search (N=2048, 512 trials) instead: the plain
 batched FP64 search over 512 trials at 18000 steps sets the target NRMSE, and
 successive halving and the surrogate-guided search are timed until they come
 within 2 % of it (value = median GPU-seconds over the surrogate-search seeds).
 
-With --fast it runs config 3 (FLIF-GL N=16384, L=1024, BA m=8) in FP64 exact and
+With --fast it runs FLIF-GL N=16384, L=1024, BA m=8 in FP64 exact and
 FP32 fast mode on the same inputs and reports both rates, raster agreement, the first
 raster divergence, max |dV| before it and each readout's NRMSE (the block size K is
 picked by timing the candidates on a twin reservoir).
@@ -147,7 +147,7 @@ def agile_cpu_baseline(budget_s=10.0):
 
 
 def run_agile(args):
-    """Config-4 hyperparameter search: the plain batched FP64 search (all 512
+    """AGILE hyperparameter search (N=2048, 512 trials, rungs 2k/6k/18k): the plain batched FP64 search (all 512
     trials at 18000 steps) sets the target; successive halving (no surrogate)
     and the surrogate-guided search (several search seeds) are timed until a
     top-rung validation NRMSE is within 2 % of it."""
@@ -231,7 +231,7 @@ def fast_k_autotune(make, u, Ks=(16, 32, 48, 64), steps=300):
 
 
 def run_batched(args):
-    """Config 2: 64 FLIF-GL reservoirs of uneven N (log-U[256, 8192]), L, alpha
+    """64 FLIF-GL reservoirs of uneven N (log-U[256, 8192]), L, alpha
     and topology (ER p=0.05 / BA m=5), N(0,1) weights rescaled to rho, on one
     13-channel input of AR(1) class sequences, advanced in time blocks of K
     steps.  The reservoirs that fit the product-wave kernel run as one
@@ -245,7 +245,7 @@ def run_batched(args):
     from spires_amd import workloads
     u, lens, labels = workloads.class_sequences()
     T = u.shape[0]
-    specs = workloads.config2_specs()
+    specs = workloads.uneven_batch_specs()
     res, bps = [], []
     for sp in specs:
         params = workloads.flif_gl_params(alpha=sp["alpha"], T_mem=sp["T_mem"])
@@ -330,7 +330,7 @@ def run_batched(args):
     for r in res:
         r.close()
     line = {
-        "metric": "batched neuron-steps/s, config 2",
+        "metric": "batched neuron-steps/s, 64 uneven FLIF-GL reservoirs",
         "value": value, "unit": "neuron-steps/s", "n_gpus": 1, "steps": steps, "higher_is_better": True,
         "dtype": "f64", "data": "synthetic",
         "config": {"workload": "64 FLIF-GL reservoirs, N log-U[256,8192] (total %d neurons), ER p=0.05 / BA m=5, "
@@ -348,11 +348,11 @@ def run_batched(args):
     print(json.dumps(line), flush=True)
 
 
-LAMBDAS = (1e-2, 1.0, 1e2)   # ridge lambdas of the config-3 readout comparison
+LAMBDAS = (1e-2, 1.0, 1e2)   # ridge lambdas of the N=16384 readout comparison
 
 
 def run_fast(args):
-    """Config 3: FLIF-GL N=16384, L=1024, alpha=0.7, Barabasi-Albert m=8, N(0,1)
+    """FLIF-GL N=16384, L=1024, alpha=0.7, Barabasi-Albert m=8, N(0,1)
     weights at rho=0.9, Mackey-Glass input, T steps, in FP64 exact mode (the
     reference's arithmetic, bit-exact with the oracle) and in fast mode (FP32
     MFMA history, split-BF16 Gram) on the same inputs.  Reports both rates,
@@ -419,7 +419,7 @@ def run_fast(args):
     fs = out["fast"]["us_per_step"] * 1e-6
     tflops = 2.0 * (L - 1) * n / fs / 1e12
     line = {
-        "metric": "fast mode vs FP64 exact, config 3",
+        "metric": "fast mode vs FP64 exact, FLIF-GL N=16384 L=1024",
         "value": out["fast"]["neuron_steps_per_s"],
         "unit": "neuron-steps/s",
         "n_gpus": 1, "steps": T, "higher_is_better": True, "dtype": "f32 history / f64 update",
@@ -450,13 +450,13 @@ def main():
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first copy 256 MiB device-to-device (counter calibration for rocprofv3 --pmc)")
     ap.add_argument("--agile", action="store_true",
-                    help="run the config-4 hyperparameter search comparison instead of the step benchmark")
+                    help="run the AGILE hyperparameter search comparison instead of the step benchmark")
     ap.add_argument("--agile-seeds", type=int, default=6)
     ap.add_argument("--fast", action="store_true",
-                    help="run the config-3 fast-mode comparison (N=16384, L=1024) instead of the step benchmark")
+                    help="run the fast-mode comparison (FLIF-GL N=16384, L=1024) instead of the step benchmark")
     ap.add_argument("--fast-steps", type=int, default=50000)
     ap.add_argument("--batched", action="store_true",
-                    help="run the config-2 batched workload (64 uneven reservoirs) instead of the step benchmark")
+                    help="run the batched workload (64 uneven reservoirs) instead of the step benchmark")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs one reservoir per device and has no "
@@ -488,7 +488,7 @@ def main():
         print(json.dumps({"copy_calibration_bytes": nbytes}), flush=True)
         del src, dst
     K, W = args.steps, args.warmup
-    cfg, csr, W_in, u = workloads.config1(T=K + W)
+    cfg, csr, W_in, u = workloads.flifgl_small_world(T=K + W)
     n = cfg[0].num_neurons
     L = 512
     r = sa.Reservoir(cfg, csr=csr, W_in=W_in)

@@ -41,7 +41,7 @@ def gaussian_reservoir(n, topology, conn, rho, params, seed, neuron="flif_gl", d
     return cfg, (rp, ci, val), W_in
 
 
-def config1(n=4096, T=20000, T_mem=512, seed=1):
+def flifgl_small_world(n=4096, T=20000, T_mem=512, seed=1):
     """FP64 exact mode, single reservoir: FLIF-GL N=4096, L=512, Watts-Strogatz
     k=20 p=0.1, N(0,1) weights at rho=0.95, gain 2.0, input U[0,0.5],
     delayed-recall targets u[n-d], d=1..100."""
@@ -98,7 +98,7 @@ def mackey_glass(T, seed=0, tau=17.0, beta=0.2, gamma=0.1, n=10, dt=0.1, every=1
     return series.copy()
 
 
-def config2_specs(count=64, seed=2):
+def uneven_batch_specs(count=64, seed=2):
     """Hyperparameters of the batched workload: N log-uniform in [256, 8192],
     topology alternating Erdos-Renyi p=0.05 / Barabasi-Albert m=5, alpha
     U[0.5, 1], T_mem in {128, 256, 512, 1024}, rho U[0.7, 1.2]."""

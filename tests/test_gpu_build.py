@@ -83,7 +83,7 @@ def test_random_er_bit_exact(n, p, ei, seed):
 
 
 def test_create_device_reservoir():
-    """A config-0-shaped reservoir (discrete LIF N=512, ER 0.1, rho 0.9) built
+    """A discrete-LIF N=512, ER p=0.1, rho 0.9 reservoir built
     on the GPU runs bit-exact against the oracle on its own weights, and its
     one-step Mackey-Glass ridge readout is as good as a rand()-built one's."""
     N, T = 512, 3000

@@ -19,7 +19,7 @@ KW = dict(din=1, dout=1, rho=0.9, gain=1.0, conn=0.1, conn_type="random", neuron
           params=[0.0, 1.0, 0.1, 0.0])
 
 
-def test_config0_full_size_bit_exact_and_ridge():
+def test_lif_er_n512_full_size_bit_exact_and_ridge():
     series = workloads.mackey_glass(T + 1, seed=0)
     u = series[:T].reshape(T, 1)
     y = series[1:T + 1].reshape(T, 1)

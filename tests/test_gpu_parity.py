@@ -111,7 +111,7 @@ def test_reset_keeps_synapse_state_and_gl_table():
 
 
 def test_explicit_csr_gaussian_weights_bit_exact():
-    """Scaled-down config-1 shape: FLIF-GL on a Watts-Strogatz ring with
+    """Scaled-down shape of the N=4096 small-world workload: FLIF-GL on a Watts-Strogatz ring with
     N(0,1) weights rescaled to rho = 0.95, gain 2.0, u ~ U[0, 0.5]."""
     n, L = 512, 64
     params = [1.0, 0.0, 0.0, 20.0, 0.8, float(L), 0.05, 2.0]
