@@ -73,10 +73,13 @@ __device__ __forceinline__ bool gather_spikes(const unsigned long long* xch, int
             }
             break;
         }
-        // Every workgroup polls the same exchange words with far-memory loads;
-        // polling about every 512 cycles instead of every 64 keeps them from
-        // becoming a hot spot (FLIF-GL N=4096, L=512: 3.55 us per step, not 3.87).
-        __builtin_amdgcn_s_sleep(8);
+        // Every workgroup polls the same exchange words with far-memory loads.
+        // With 128 or more words (and as many polling workgroups), polling about
+        // every 512 cycles instead of every 64 keeps them from becoming a hot
+        // spot (FLIF-GL N=4096, L=512: 3.5 us per step, not 3.9); smaller
+        // reservoirs keep the short interval.
+        if (nw >= 128) __builtin_amdgcn_s_sleep(8);
+        else __builtin_amdgcn_s_sleep(1);
     }
 #pragma unroll
     for (int q = 0; q < kMaxWordsPerLane; ++q)
