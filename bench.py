@@ -451,7 +451,7 @@ def main():
                     help="first copy 256 MiB device-to-device (counter calibration for rocprofv3 --pmc)")
     ap.add_argument("--agile", action="store_true",
                     help="run the AGILE hyperparameter search comparison instead of the step benchmark")
-    ap.add_argument("--agile-seeds", type=int, default=6)
+    ap.add_argument("--agile-seeds", type=int, default=12)
     ap.add_argument("--fast", action="store_true",
                     help="run the fast-mode comparison (FLIF-GL N=16384, L=1024) instead of the step benchmark")
     ap.add_argument("--fast-steps", type=int, default=50000)

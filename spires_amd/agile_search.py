@@ -324,9 +324,9 @@ def _levy_steps(rng, count, dim, mu=1.3, lo=1e-3, hi=0.3):
     return d * s[:, None]
 
 
-def surrogate_search(ev, rungs, target=None, budget=512, warm=96, wave=32, promote=8, pool=2048, seed=1,
-                     epochs=3000, round_epochs=500, lr=1e-3, reseed=4, explore=8, top_weight=4, clip=2.0,
-                     novelty=0.5, log=None):
+def surrogate_search(ev, rungs, target=None, budget=512, warm=96, wave=32, promote=4, pool=2048, seed=1,
+                     epochs=3000, round_epochs=500, lr=1e-3, reseed=6, reseed_top=3, reseed_direct=True,
+                     explore=8, top_weight=4, clip=2.0, novelty=0.5, levy_hi=0.15, log=None):
     """Surrogate-guided search with the fidelity rungs.
 
     Every trial first runs to the middle rung (the lowest rung is a checkpoint
@@ -339,8 +339,11 @@ def surrogate_search(ev, rungs, target=None, budget=512, warm=96, wave=32, promo
     best by measured middle-rung score and half the best by the surrogate's
     top-rung prediction; the others are pruned.  The first wave holds `warm`
     trials from a Latin hypercube; each later wave holds `wave` proposals:
-      - `reseed` fresh weight seeds for the two best hyperparameter sets so far
-        (a trial's score varies strongly with its weight draw);
+      - `reseed` fresh weight seeds for the `reseed_top` best hyperparameter
+        sets so far (a trial's score varies strongly with its weight draw);
+        with `reseed_direct` they go to the top rung without the middle-rung
+        gate (whose score ranks the best low-lambda sets poorly: 3900 training
+        rows for N=2048 features overfit them);
       - `explore` global samples, half unscreened and half the best of `pool`
         by predicted score minus `novelty` x distance to the nearest trial run;
       - the rest from its ranking of `pool` Levy-flight moves around the eight
@@ -380,8 +383,9 @@ def surrogate_search(ev, rungs, target=None, budget=512, warm=96, wave=32, promo
     def rank(v):
         return np.argsort(np.argsort(v))
 
-    def run_wave(new, ep):
+    def run_wave(new, ep, forced=0):
         nonlocal best
+        # the first `forced` trials go to the top rung whatever their middle-rung score
         for a in range(0, len(new), 32):
             part = new[a:a + 32]
             ev.prefetch(new[a + 32:a + 64])
@@ -391,8 +395,10 @@ def surrogate_search(ev, rungs, target=None, budget=512, warm=96, wave=32, promo
         train(ep)
         pred = net.predict(encode(np.array([t.theta for t in new]), f_max, f_min, f_max))
         meas = np.array([t.scores[mid] for t in new])
-        chosen = list(np.argsort(meas, kind="stable")[:promote // 2])
-        chosen += [i for i in np.argsort(pred, kind="stable") if i not in chosen][:promote - len(chosen)]
+        meas[:forced] = -np.inf
+        pred[:forced] = -np.inf
+        chosen = list(np.argsort(meas, kind="stable")[:forced + promote // 2])
+        chosen += [i for i in np.argsort(pred, kind="stable") if i not in chosen][:forced + promote - len(chosen)]
         up = [new[i] for i in chosen]
         ev.release([t for t in new if t not in up])
         ev.advance(up, f_max)
@@ -412,7 +418,7 @@ def surrogate_search(ev, rungs, target=None, budget=512, warm=96, wave=32, promo
         rounds += 1
         train(round_epochs)
         inc = sorted(done, key=lambda d: d[1])
-        pick = [inc[q % min(2, len(inc))][0] for q in range(reseed)]
+        pick = [inc[q % min(reseed_top, len(inc))][0] for q in range(reseed)]
         glob = sample_trials(pool, int(rng.integers(1 << 31)))
         pg = net.predict(encode(glob, f_max, f_min, f_max))
         # screened global picks favour unexplored regions: predicted score minus
@@ -424,10 +430,11 @@ def surrogate_search(ev, rungs, target=None, budget=512, warm=96, wave=32, promo
         pick += list(glob[:explore // 2])                                   # unscreened
         pick += [glob[i] for i in np.argsort(acq) if i >= explore // 2][:explore - explore // 2]
         base = to_unit(np.array([d[0] for d in inc[:8]]))
-        loc = from_unit(base[rng.integers(len(base), size=pool)] + _levy_steps(rng, pool, 6, hi=0.15))
+        loc = from_unit(base[rng.integers(len(base), size=pool)] + _levy_steps(rng, pool, 6, hi=levy_hi))
         pl = net.predict(encode(loc, f_max, f_min, f_max))
         pick += [loc[i] for i in np.argsort(pl)[:max(0, wave - len(pick))]]
-        run_wave([Trial(th, s) for th, s in zip(pick, seeds(len(pick)))], round_epochs)
+        run_wave([Trial(th, s) for th, s in zip(pick, seeds(len(pick)))], round_epochs,
+                 forced=reseed if reseed_direct else 0)
         count += len(pick)
         if log:
             log(f"surrogate round {rounds}: {count} trials, best {best:.4f} t {time.perf_counter() - t0:.1f}s "
