@@ -162,6 +162,16 @@ void *spires_dev_stream(spires_dev *g);
 size_t spires_dev_steps(const spires_dev *g);
 const char *spires_dev_last_error(void);
 
+/* coarse_grain_reservoir (reservoir.c:782-967) on the current device: the
+ * reference's greedy merge passes over the CSR matrix (rp, ci, val) with
+ * threshold thr >= 0, in the reference's order and arithmetic.  W_in [n x din],
+ * W_out [dout x n] and V [n] are merged in place (dead neurons zeroed);
+ * alive[n] marks the survivors; the surviving neurons' matrix comes back as a
+ * malloc'd CSR over *n_out neurons (ascending columns, zeros dropped). */
+int  spires_dev_coarse_grain(int n, const int64_t *row_ptr, const int32_t *col_idx, const double *values,
+                             double thr, int din, double *W_in, int dout, double *W_out, double *V,
+                             int *alive, int *n_out, int64_t **row_ptr_out, int32_t **col_idx_out,
+                             double **values_out, size_t *nnz_out, int *passes);
 /* Reservoir construction on the current device.  Erdos-Renyi W (edge i->j,
  * i != j, with probability p; magnitude U(0,1], positive with probability ei;
  * SplitMix64 of (seed, i, j), so one seed gives one matrix on any device)

@@ -469,65 +469,43 @@ spires_status spires_train_rls(spires_reservoir* r, const double* input_series, 
     return dev_status(rc);
 }
 
-// coarse_grain_reservoir, reservoir.c:782-967 (host transform; the new
-// reservoir is not rescaled and keeps the merged weights).
+// coarse_grain_reservoir, reservoir.c:782-967: the merge passes run on the
+// device (spires_dev_coarse_grain, same order and arithmetic); the new
+// reservoir is not rescaled and keeps the merged weights.
 spires_status spires_coarse_grain(const spires_reservoir* rc_, double thr, spires_reservoir** out_r) {
     if (!rc_ || !rc_->dev || !out_r) return SPIRES_ERR_INVALID_ARG;
     spires_reservoir* r = const_cast<spires_reservoir*>(rc_);
     const size_t n = r->cfg.num_neurons, din = r->cfg.num_inputs, dout = r->cfg.num_outputs;
-    std::vector<double> W(n * n, 0.0), Wi = r->W_in, Wo(n * dout), V(n);
-    for (size_t i = 0; i < n; ++i)
-        for (int64_t k = r->W.rp[i]; k < r->W.rp[i + 1]; ++k) W[i * n + r->W.ci[k]] = r->W.val[k];
+    if (n > (size_t)INT32_MAX) return SPIRES_ERR_INVALID_ARG;
+    std::vector<double> Wi = r->W_in, Wo(n * dout), V(n);
     if (dout && spires_dev_get_readout(r->dev, Wo.data())) return dev_status(SPIRES_DEV_EINTERNAL);
     if (spires_dev_read_state(r->dev, V.data())) return dev_status(SPIRES_DEV_EINTERNAL);
     std::vector<int> alive(n, 1);
-    bool merged = true;
-    while (merged) {
-        merged = false;
-        for (size_t i = 0; i < n; ++i)
-            for (size_t j = i + 1; j < n; ++j) {
-                if (W[i * n + j] <= thr && W[j * n + i] <= thr) continue;
-                merged = true;
-                V[i] = (V[i] + V[j]) / 2.0;
-                V[j] = 0.0;
-                alive[j] = 0;
-                for (size_t k = 0; k < n; ++k) {
-                    if (k == i || k == j) continue;
-                    const double a = W[i * n + k], b = W[j * n + k];
-                    const double nw = (a * b == 0.0) ? a + b : (a + b) / 2.0;
-                    W[i * n + k] = nw;
-                    W[k * n + i] = nw;
-                }
-                for (size_t k = 0; k < din; ++k) {
-                    const double a = Wi[i * din + k], b = Wi[j * din + k];
-                    Wi[i * din + k] = (a * b == 0.0) ? a + b : (a + b) / 2.0;
-                    Wi[j * din + k] = 0.0;
-                }
-                for (size_t o = 0; o < dout; ++o) { Wo[o * n + i] += Wo[o * n + j]; Wo[o * n + j] = 0.0; }
-                for (size_t k = 0; k < n; ++k) { W[j * n + k] = 0.0; W[k * n + j] = 0.0; }
-            }
-    }
-    size_t ns = 0;
-    std::vector<size_t> map(n, (size_t)-1);
-    for (size_t i = 0; i < n; ++i) if (alive[i]) map[i] = ns++;
-    std::vector<int64_t> rp(ns + 1, 0);
-    std::vector<int32_t> ci;
-    std::vector<double> val, Win(ns * din), Won(ns * dout), Vn(ns);
+    int ns = 0;
+    int64_t* rp = nullptr;
+    int32_t* ci = nullptr;
+    double* val = nullptr;
+    size_t nnz = 0;
+    spires_status s = dev_status(spires_dev_coarse_grain((int)n, r->W.rp.data(), r->W.ci.data(), r->W.val.data(), thr,
+                                                         (int)din, Wi.data(), (int)dout, Wo.data(), V.data(),
+                                                         alive.data(), &ns, &rp, &ci, &val, &nnz, nullptr));
+    if (s != SPIRES_OK) return s;
+    std::vector<double> Win((size_t)ns * din), Won((size_t)ns * dout), Vn(ns);
+    size_t a = 0;
     for (size_t i = 0; i < n; ++i) {
         if (!alive[i]) continue;
-        const size_t a = map[i];
-        for (size_t j = 0; j < n; ++j)
-            if (alive[j] && W[i * n + j] != 0.0) { ci.push_back((int32_t)map[j]); val.push_back(W[i * n + j]); }
-        rp[a + 1] = (int64_t)ci.size();
         for (size_t k = 0; k < din; ++k) Win[a * din + k] = Wi[i * din + k];
         for (size_t o = 0; o < dout; ++o) Won[o * ns + a] = Wo[o * n + i];
         Vn[a] = V[i];
+        ++a;
     }
     spires_reservoir_config c = r->cfg;
-    c.num_neurons = ns;
+    c.num_neurons = (size_t)ns;
     spires_reservoir* q = nullptr;
-    spires_status s = spires_ext_create_csr(&c, val.size(), rp.data(), ci.data(), val.data(), Win.data(),
-                                            Won.data(), &q);
+    s = spires_ext_create_csr(&c, nnz, rp, ci, val, Win.data(), Won.data(), &q);
+    std::free(rp);
+    std::free(ci);
+    std::free(val);
     if (s != SPIRES_OK) return s == SPIRES_ERR_ALLOC ? SPIRES_ERR_ALLOC : SPIRES_ERR_INTERNAL;
     if (spires_dev_set_state(q->dev, Vn.data())) { spires_reservoir_destroy(q); return SPIRES_ERR_INTERNAL; }
     *out_r = q;
