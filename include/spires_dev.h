@@ -110,7 +110,8 @@ int  spires_dev_timeblock_run(spires_dev_timeblock *tb, long long blocks);
 int  spires_dev_timeblock_launches(const spires_dev_timeblock *tb);
 void spires_dev_timeblock_destroy(spires_dev_timeblock *tb);
 /* Exact-mode launch plan of a run of this reservoir alone: info[0] kernel
- * (0 per-step launches, 1 single-wave persistent, 2 product-wave persistent),
+ * (0 per-step launches, 1 single-wave persistent, 2 product-wave persistent,
+ * 3 product-wave persistent with a window of product chunks in LDS),
  * [1] neurons per workgroup, [2] workgroups, [3] row pull (0 edge list,
  * 1 column bitmaps), [4] recurrent weights held in LDS, [5] LDS bytes per
  * workgroup; [6] neurons per workgroup in a batched launch (0: cannot batch). */

@@ -42,6 +42,7 @@ struct PersistArgs {
     int n, din, L, G, wpg, micro, stype;
     int ne_lds;                   // LDS capacity for CSR edges per workgroup (0: read CSR from HBM)
     int rpm, cw;                  // row pull: 0 edge list, 1 spike-word bitmaps (cw words per scan part)
+    int win;                      // k_persist_tp: product window in 32-lag chunks (0: whole rows)
     long long st0;                // micro-step index since reset at launch start
     long long t0, T;              // macro steps [t0, t0+T) of the resident input series
     double gain, dt;
@@ -462,7 +463,7 @@ __host__ __device__ __forceinline__ int tp_row_stride(int L) { return 32 * ((L -
 template <int R_>
 struct RoleTag { static constexpr int value = R_; };
 
-template <int NB>
+template <int NB, bool WIN>
 __device__ __forceinline__ void tp_body(const PersistArgs& a, const int g) {
     constexpr int C = kTpChunk;
     constexpr int PARTS = 64 * kTpProducers / NB;   // producer lanes per neuron
@@ -475,9 +476,11 @@ __device__ __forceinline__ void tp_body(const PersistArgs& a, const int g) {
     const int nloc = min(NB, a.n - i0);
     const int n = a.n, L = a.L, wpg = a.wpg, nw = a.G * wpg;
     const int PS = tp_row_stride(L);
+    const int W = WIN ? a.win : 0;                     // product window (chunks); 0: whole rows
+    const int PSr = W ? 32 * W + 2 : PS;               // product row stride (2*PSr = 4 mod 32 dwords)
     // ---- LDS layout
-    double* P = lds;                                   // [NB][PS]   P[r][k]
-    double* coef_s = P + (size_t)NB * PS;              // [PS]       zero past L-1
+    double* P = lds;                                   // [NB][PSr]  P[r][k], or chunk slots when W > 0
+    double* coef_s = P + (size_t)NB * PSr;             // [PS]       zero past L-1
     double* stage = coef_s + PS;                       // [2][NB]    V of the last two steps
     double* rec_s = stage + 2 * NB;                    // [2][NB]
     double* ext_s = rec_s + 2 * NB;                    // [2][NB]
@@ -492,7 +495,11 @@ __device__ __forceinline__ void tp_body(const PersistArgs& a, const int g) {
     unsigned short* list_s = (unsigned short*)(pb_s + 64);   //    [ne + kBitmapBatch]
     __shared__ int abort_s;
     __shared__ unsigned cons_s;
-    if (tid == 0) { abort_s = 0; cons_s = 0u; }
+    // windowed products: chunks produced by each producer wave and tails done
+    // (cons_s then counts chunks consumed by the chain, over all steps)
+    __shared__ unsigned prod_s[kTpProducers];
+    __shared__ unsigned tail_s;
+    if (tid == 0) { abort_s = 0; cons_s = 0u; tail_s = 0u; }
     long long st0 = a.st0, t0 = a.t0;
     const uint8_t* S_in = a.S;
     uint8_t* S_out = a.S_out;
@@ -576,17 +583,23 @@ __device__ __forceinline__ void tp_body(const PersistArgs& a, const int g) {
     const int part = pw * (64 / NB) + lane / NB;
     int head = head0;
     int lim = st0 < L ? (int)st0 : L - 1;
-    // products of the launch's first step, all from the HBM ring
+    // products of the launch's first step (its first W chunks when windowed), all from the HBM ring
+    {
+        const int nk0 = lim >= 2 ? lim - 1 : 0;
+        const int pre0 = W ? min(W, (nk0 + C - 1) / C) : 0;
+        if (W && tid < kTpProducers) prod_s[tid] = (unsigned)pre0;
+    }
     if (producer) {
         const int nk = lim >= 2 ? lim - 1 : 0;
-        for (int k0 = 2 + part * PER; k0 < 2 + nk; k0 += C) {
+        const int klim = W ? min(nk, W * C) : nk;
+        for (int k0 = 2 + part * PER; k0 < 2 + klim; k0 += C) {
 #pragma unroll
             for (int j = 0; j < PER; j += 2) {
                 const int k = k0 + j;
                 const int ka = k < L ? k : L - 1, kb = k + 1 < L ? k + 1 : L - 1;
                 int sa = head - ka; if (sa < 0) sa += L;
                 int sbb = head - kb; if (sbb < 0) sbb += L;
-                *reinterpret_cast<double2*>(P + (size_t)r * PS + k) =
+                *reinterpret_cast<double2*>(P + (size_t)r * PSr + k) =
                     make_double2(coef_s[k] * a.ring[(size_t)sa * n + ir], coef_s[k + 1] * a.ring[(size_t)sbb * n + ir]);
             }
         }
@@ -594,9 +607,10 @@ __device__ __forceinline__ void tp_body(const PersistArgs& a, const int g) {
     __syncthreads();
 
     const double c1 = coef_s[1];
-    const double2* prow = reinterpret_cast<const double2*>(P + (size_t)r * PS);   // pair j: lags 2j, 2j+1
+    const double2* prow = reinterpret_cast<const double2*>(P + (size_t)r * PSr);   // pair j: lags 2j, 2j+1
     long long st = st0;
     unsigned m = 0;                                    // micro-step index within the launch
+    unsigned gb = 0;                                   // chunks of the steps before this one (windowed)
     // One loop per wave role: with the roles as branches of a single loop, copies
     // of the producers' in-flight load registers sit on every path and make the
     // chain wave wait for its own global stores (the store counter is shared).
@@ -631,7 +645,7 @@ __device__ __forceinline__ void tp_body(const PersistArgs& a, const int g) {
                             }
                         }
                     };
-                    if (lane < NB) {   // lanes past NB would only repeat the rows of lanes 0..NB-1
+                    if (!WIN && lane < NB) {   // lanes past NB would only repeat the rows of lanes 0..NB-1
                         if (nch > 0) ld(bA, 0);
                         int c = 0;
                         for (; c + 2 <= nch; c += 2) {
@@ -645,6 +659,59 @@ __device__ __forceinline__ void tp_body(const PersistArgs& a, const int g) {
                         if (c < nch) {
                             add(bA, nk - c * C);
                             lds_counter_store(&cons_s, m * 64u + (unsigned)c + 1u);
+                        }
+                    } else if (WIN && lane < NB) {
+                        // windowed: chunk c of this step is chunk gb+c of the stream, in slot
+                        // (gb+c) mod W; it is read once every producer wave has written it, and
+                        // its slot is released (cons_s) after its adds
+                        bool stalled = false;
+                        auto wait_prod = [&](unsigned need) {
+                            long long spins = 0;
+                            while (!stalled) {
+                                asm volatile("" ::: "memory");
+                                bool ok = true;
+#pragma unroll
+                                for (int q = 0; q < kTpProducers; ++q)
+                                    ok = ok && (int)(lds_counter_load(&prod_s[q]) - need) >= 0;
+                                if (ok) break;
+                                if (++spins > kTpSpinLimit) {
+                                    stalled = true;
+                                    __hip_atomic_store(&abort_s, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                                    __hip_atomic_store(a.abort_flag, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                                }
+                                __builtin_amdgcn_s_sleep(1);
+                            }
+                            asm volatile("" ::: "memory");
+                        };
+                        auto ldw = [&](double2* buf, int c) {
+                            const int sl = (int)((gb + (unsigned)c) % (unsigned)W);
+#pragma unroll
+                            for (int j = 0; j < C / 2; ++j) buf[j] = prow[1 + (C / 2) * sl + j];
+                        };
+                        auto release = [&](unsigned v) {
+                            asm volatile("" ::: "memory");
+                            lds_counter_store(&cons_s, v);
+                        };
+                        if (nch > 0) {
+                            wait_prod(gb + 1u);
+                            ldw(bA, 0);
+                        }
+                        int c = 0;
+                        for (; c + 2 <= nch; c += 2) {
+                            wait_prod(gb + (unsigned)c + 2u);
+                            ldw(bB, c + 1);
+                            add(bA, C);
+                            release(gb + (unsigned)c + 1u);
+                            if (c + 2 < nch) {
+                                wait_prod(gb + (unsigned)c + 3u);
+                                ldw(bA, c + 2);
+                            }
+                            add(bB, (c + 1 == nch - 1) ? nk - (c + 1) * C : C);
+                            release(gb + (unsigned)c + 2u);
+                        }
+                        if (c < nch) {
+                            add(bA, nk - c * C);
+                            release(gb + (unsigned)c + 1u);
                         }
                     }
                     const double leak = -(Vprev - p[2]) / p[3];   // before the barrier: the wait hides it
@@ -671,7 +738,11 @@ __device__ __forceinline__ void tp_body(const PersistArgs& a, const int g) {
                             if (rec_S) rec_S[(size_t)t * n + i] = spk ? 1 : 0;
                         }
                     }
-                    lds_counter_store(&cons_s, (m + 1u) * 64u);   // tail done: V_st staged
+                    if constexpr (!WIN) lds_counter_store(&cons_s, (m + 1u) * 64u);   // tail done: V_st staged
+                    else {
+                        asm volatile("" ::: "memory");   // V_st staged before the signal
+                        lds_counter_store(&tail_s, m + 1u);
+                    }
                 } else if constexpr (R == 4) {
                     // ---------------- exchange, row pull, external input
                     double uin[kMaxDinReg];
@@ -720,7 +791,7 @@ __device__ __forceinline__ void tp_body(const PersistArgs& a, const int g) {
                     }
                     __syncthreads();
                     if (__builtin_amdgcn_readfirstlane(abort_s)) break;
-                } else {
+                } else if constexpr (!WIN) {
                     // ---------------- products of step st+1, behind the chain of step st
                     const int headn = head + 1 == L ? 0 : head + 1;
                     const int limn = lim < L - 1 ? lim + 1 : L - 1;
@@ -787,9 +858,93 @@ __device__ __forceinline__ void tp_body(const PersistArgs& a, const int g) {
                     if (c + 2 < nchn) put(vC, c + 2);
                     __syncthreads();
                     if (__builtin_amdgcn_readfirstlane(abort_s)) break;
+                } else {
+                    // ---------------- windowed products: the chunks of step st past the first W
+                    // (into the slots the chain releases), then the first W of step st+1.  Chunk g
+                    // of the stream goes to slot g mod W once the chain has released chunk g-W.
+                    const int headn = head + 1 == L ? 0 : head + 1;
+                    const int limn = lim < L - 1 ? lim + 1 : L - 1;
+                    const int nkn = limn >= 2 ? limn - 1 : 0;
+                    const int nchn = (nkn + C - 1) / C;
+                    const int pre = min(W, nch), qa = nch - pre;
+                    const int Q = qa + min(W, nchn);
+                    // item q: chunk pre+q of step st (q < qa) or chunk q-qa of step st+1.  The
+                    // ring holds V_{st-k} at (head - k) mod L for step st and at (headn - k)
+                    // for step st+1; every value read is final since the last barrier except
+                    // lag 2 of step st+1 (V_{st-1}), which comes from the stage.
+                    auto ldq = [&](double* v, int q) {
+                        const bool nx = q >= qa;
+                        const int cc = nx ? q - qa : pre + q;
+                        const int sb0 = (nx ? headn : head) - (2 + cc * C + part * PER);
+#pragma unroll
+                        for (int j = 0; j < PER; ++j) {
+                            int sl = sb0 - j;
+                            sl += sl < 0 ? L : 0;
+                            sl += sl < 0 ? L : 0;
+                            v[j] = a.ring[(size_t)sl * n + ir];
+                        }
+                    };
+                    bool stalled = false;
+                    auto spin_until = [&](const unsigned* ctr, unsigned need) {
+                        long long spins = 0;
+                        while (!stalled) {
+                            asm volatile("" ::: "memory");
+                            if ((int)(lds_counter_load(ctr) - need) >= 0) break;
+                            if (++spins > kTpSpinLimit) {
+                                stalled = true;
+                                __hip_atomic_store(&abort_s, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                                __hip_atomic_store(a.abort_flag, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                            }
+                            __builtin_amdgcn_s_sleep(1);
+                        }
+                        asm volatile("" ::: "memory");
+                    };
+                    auto putq = [&](const double* v, int q) {
+                        const bool nx = q >= qa;
+                        const int cc = nx ? q - qa : pre + q;
+                        const unsigned gq = gb + (unsigned)(nx ? nch + cc : cc);
+                        double2 pr[PER / 2];
+#pragma unroll
+                        for (int j = 0; j < PER / 2; ++j) {
+                            const int k = 2 + cc * C + part * PER + 2 * j;
+                            pr[j] = make_double2(coef_s[k] * v[2 * j], coef_s[k + 1] * v[2 * j + 1]);
+                        }
+                        spin_until(&cons_s, gq + 1u - (unsigned)W);   // slot released by chunk gq-W
+                        if (nx && cc == 0 && part == 0) {
+                            spin_until(&tail_s, m);                    // V_{st-1} staged
+                            pr[0].x = coef_s[2] * stage[(int)((st - 1) & 1) * NB + r];
+                        }
+                        double2* dst = reinterpret_cast<double2*>(P + (size_t)r * PSr + 2 +
+                                                                  (size_t)(gq % (unsigned)W) * C + part * PER);
+#pragma unroll
+                        for (int j = 0; j < PER / 2; ++j) dst[j] = pr[j];
+                        asm volatile("" ::: "memory");
+                        lds_counter_store(&prod_s[pw], gq + 1u);
+                    };
+                    double vA[PER], vB[PER], vC[PER], vD[PER];
+                    if (Q > 0) ldq(vA, 0);
+                    if (Q > 1) ldq(vB, 1);
+                    if (Q > 2) ldq(vC, 2);
+                    int q = 0;
+                    for (; q + 4 <= Q; q += 4) {
+                        ldq(vD, q + 3);
+                        putq(vA, q);
+                        if (q + 4 < Q) ldq(vA, q + 4);
+                        putq(vB, q + 1);
+                        if (q + 5 < Q) ldq(vB, q + 5);
+                        putq(vC, q + 2);
+                        if (q + 6 < Q) ldq(vC, q + 6);
+                        putq(vD, q + 3);
+                    }
+                    if (q < Q) putq(vA, q);
+                    if (q + 1 < Q) putq(vB, q + 1);
+                    if (q + 2 < Q) putq(vC, q + 2);
+                    __syncthreads();
+                    if (__builtin_amdgcn_readfirstlane(abort_s)) break;
                 }
                 head = head + 1 == L ? 0 : head + 1;
                 if (lim < L - 1) ++lim;
+                gb += (unsigned)nch;
             }
             if (__builtin_amdgcn_readfirstlane(abort_s)) break;
         }
@@ -806,7 +961,15 @@ __device__ __forceinline__ void tp_body(const PersistArgs& a, const int g) {
 
 template <int NB>
 __global__ void __launch_bounds__(kTpThreads) k_persist_tp(PersistArgs a) {
-    tp_body<NB>(a, blockIdx.x);
+    tp_body<NB, false>(a, blockIdx.x);
+}
+
+// The same with a window of product chunks (PersistArgs::win), for rows too
+// long to keep whole in LDS; a separate kernel so that the whole-row kernel
+// keeps its registers.
+template <int NB>
+__global__ void __launch_bounds__(kTpThreads) k_persist_tpw(PersistArgs a) {
+    tp_body<NB, true>(a, blockIdx.x);
 }
 
 // Several reservoirs in one launch: workgroup b runs local workgroup
@@ -816,7 +979,7 @@ template <int NB>
 __global__ void __launch_bounds__(kTpThreads) k_persist_tp_batch(const PersistArgs* __restrict__ table,
                                                                  const int2* __restrict__ wg_map) {
     const int2 mw = wg_map[blockIdx.x];
-    tp_body<NB>(table[mw.x], mw.y);
+    tp_body<NB, false>(table[mw.x], mw.y);
 }
 
 namespace {
@@ -825,6 +988,7 @@ struct Plan {
     int nb = 0, G = 0, ring_lds = 0, ne_lds = 0;
     int tp = 0;                 // FLIF-GL with the trailing product stage (k_persist_tp)
     int rpm = 0, cw = 0;        // row pull over spike-word bitmaps (k_persist_tp only)
+    int win = 0;                // k_persist_tp product window in chunks (0: whole rows)
     size_t lds = 0;
 };
 
@@ -854,24 +1018,40 @@ bool persist_plan(spires_dev* g, int ncu, Plan* pl, bool batch = false) {
         if (base > cap) return false;
         const int64_t me = max_edges(nb);
         pl->ne_lds = (base + (size_t)me * 12 <= cap) ? (int)std::max<int64_t>(me, 1) : 0;
-        pl->nb = nb; pl->G = G; pl->ring_lds = rl;
+        pl->nb = nb; pl->G = G; pl->ring_lds = rl; pl->tp = 0; pl->rpm = 0; pl->cw = 0; pl->win = 0;
         pl->lds = base + (size_t)pl->ne_lds * 12;
         return true;
     };
     // Product-wave kernel.  Row pull: spike-word bitmaps when a workgroup's rows
     // hold more edges than half the words a bitmap scan reads, else the edge
     // list; weights in LDS when they fit ("fast"), else read from HBM.
+    // Whole product rows when they fit in LDS; otherwise a window of W 32-lag
+    // chunks per row (the largest of 16, 8, 4 that fits), refilled by the
+    // product waves while the chain runs.
     auto fill_tp = [&](int nb, bool fast_only) -> bool {
         const int G = (n + nb - 1) / nb;
         const int wpg = (nb + 31) / 32;
         if (L < 2 || G > ncu || G * wpg > 512) return false;
         const size_t PS = (size_t)tp_row_stride(L);
-        const size_t base = PS * nb * 8 + PS * 8 + 6 * (size_t)nb * 8 + (size_t)G * wpg * 4 + 64;
-        if (base > cap) return false;
+        const size_t rest = PS * 8 + 6 * (size_t)nb * 8 + (size_t)G * wpg * 4 + 64;
+        size_t base = PS * nb * 8 + rest;
+        int win = 0;
+        if (base > cap) {
+            if (batch) return false;   // batched launches take whole rows only
+            const int nch_max = (L - 2 + kTpChunk - 1) / kTpChunk;
+            for (int w : {16, 8, 4}) {
+                const size_t b = (size_t)(32 * w + 2) * nb * 8 + rest;
+                if (w < nch_max && b <= cap) { win = w; base = b; break; }
+            }
+            if (!win) return false;
+        } else if (nb == 64) {
+            return false;   // whole rows at 64 neurons per workgroup: not instantiated (32 fits first)
+        }
         const int64_t me = max_edges(nb);
         const int nwp = (n + 31) / 32;
-        pl->nb = nb; pl->G = G; pl->ring_lds = 0; pl->tp = 1; pl->rpm = 0; pl->cw = 0;
-        const bool dense = g->stype == ST_SIMPLE && me + kBitmapBatch < 65536 && 2 * me > (int64_t)nb * nwp;
+        pl->nb = nb; pl->G = G; pl->ring_lds = 0; pl->tp = 1; pl->rpm = 0; pl->cw = 0; pl->win = win;
+        const bool dense = nb <= 32 && g->stype == ST_SIMPLE && me + kBitmapBatch < 65536 &&
+                           2 * me > (int64_t)nb * nwp;
         if (dense) {
             const size_t bm = ((size_t)(nb + (nb == 32 ? 0 : 1)) * nwp * 4 + 512 + ((size_t)me + kBitmapBatch) * 2 + 7) /
                               8 * 8;
@@ -897,11 +1077,12 @@ bool persist_plan(spires_dev* g, int ncu, Plan* pl, bool batch = false) {
         // that keeps the row pull in LDS), then the single-wave chain with the
         // history in LDS, then in HBM; path 3 / 4 force the single-wave HBM / LDS variants.
         if (g->path == 0 || g->path == 2) {
-            const int order[2] = {batch ? 32 : 16, batch ? 16 : 32};
+            // 64 neurons per workgroup only for a reservoir launched alone with a window
+            const int order[3] = {batch ? 32 : 16, batch ? 16 : 32, batch ? 0 : 64};
             for (int nb : order)
-                if (fill_tp(nb, true)) return true;
+                if (nb && fill_tp(nb, true)) return true;
             for (int nb : order)
-                if (fill_tp(nb, false)) return true;
+                if (nb && fill_tp(nb, false)) return true;
         }
         if (g->path != 3)
             for (int nb : {32, 16})
@@ -989,7 +1170,7 @@ PersistArgs make_args(spires_dev* g, const Plan& pl, const double* u_series, lon
     memset(&a, 0, sizeof(a));
     a.n = g->n; a.din = g->din; a.L = g->L; a.G = pl.G; a.wpg = (pl.nb + 31) / 32; a.micro = g->micro;
     a.stype = g->stype; a.ne_lds = pl.ne_lds;
-    a.rpm = pl.rpm; a.cw = pl.cw; a.rbm = g->rbm;
+    a.rpm = pl.rpm; a.cw = pl.cw; a.rbm = g->rbm; a.win = pl.win;
     a.st0 = g->step; a.t0 = t0; a.T = T;
     a.gain = g->d.gain; a.dt = g->d.dt;
     memcpy(a.np, g->d.np, sizeof(a.np));
@@ -1028,7 +1209,11 @@ int launch_persist(spires_dev* g, const double* u_series, long long t0, long lon
     const void* fn = nullptr;
     int threads = kPersistThreads;
     if (pl.tp) {
-        fn = pl.nb == 32 ? (const void*)k_persist_tp<32> : (const void*)k_persist_tp<16>;
+        if (pl.win)
+            fn = pl.nb == 64 ? (const void*)k_persist_tpw<64>
+                 : pl.nb == 32 ? (const void*)k_persist_tpw<32> : (const void*)k_persist_tpw<16>;
+        else
+            fn = pl.nb == 32 ? (const void*)k_persist_tp<32> : (const void*)k_persist_tp<16>;
         threads = kTpThreads;
     } else {
         switch (g->ntype) {
@@ -1116,7 +1301,7 @@ extern "C" int spires_dev_plan_info(spires_dev* g, int* info) {
     sdev::Plan pl, pb;
     const int ncu = sdev::num_cus();
     if (g->mode != SPIRES_DEV_MODE_EXACT || g->path == 1 || !sdev::persist_plan(g, ncu, &pl)) return 0;
-    info[0] = pl.tp ? 2 : 1;
+    info[0] = pl.tp ? (pl.win ? 3 : 2) : 1;
     info[1] = pl.nb;
     info[2] = pl.G;
     info[3] = pl.rpm;
