@@ -29,7 +29,7 @@ raster divergence, max |dV| before it and each readout's NRMSE (the block size K
 picked by timing the candidates on a twin reservoir).
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--copy-calibration]
-                       [--agile [--agile-seeds S]] [--fast [--fast-steps T]] [--batched]
+                       [--agile [--agile-seeds S]] [--fast [--fast-steps T]] [--batched] [--readout]
 """
 import argparse
 import ctypes as C
@@ -441,6 +441,53 @@ def run_fast(args):
     print(json.dumps(line), flush=True)
 
 
+def run_readout(args):
+    """Ridge readout of the FLIF-GL N=4096 L=512 small-world reservoir: 100
+    delayed-recall targets u[t-d], d=1..100, fitted on all T=20000 recorded
+    states with lambda=1e-5 (train_output_ridge_regression records every step,
+    reservoir.c:573-679).  A = X'X + lambda I and B = X'Y are formed on the
+    device from the state record in HBM (655 MB), then one FP64 Cholesky for all
+    100 outputs and two refinement steps.  Reports the fit time, its FP64 rate
+    against the MFMA peak and the relative normal-equation residual."""
+    import numpy as np
+    import spires_amd as sa
+    from spires_amd import workloads
+    n, K, D, lam = 4096, 20000, 100, 1e-5
+    cfg, csr, W_in, u = workloads.flifgl_small_world(T=K)
+    Y = np.ascontiguousarray(workloads.delayed_recall_targets(u, list(range(1, D + 1))))
+    r = sa.Reservoir(cfg, csr=csr, W_in=W_in)
+    r.upload_inputs(u)
+    r.reserve_records(K, sa.REC_STATE)
+    r.run_resident(0, K, sa.REC_STATE)
+    r.sync()
+    r.fit_ridge_records(0, K, Y, lam)          # handle and workspace set-up
+    times, flags, resid = [], None, None
+    for _ in range(3):
+        t0 = time.perf_counter()
+        flags, resid = r.fit_ridge_records(0, K, Y, lam)
+        times.append(time.perf_counter() - t0)
+    t = min(times)
+    # X'X (symmetric: N^2 T), X'Y, Cholesky N^3/3, three solve pairs and two residual products
+    flop = n * n * K + 2.0 * n * K * D + n ** 3 / 3.0 + 3 * 2.0 * n * n * D + 2 * 2.0 * n * n * D
+    W = r.W_out()
+    line = {
+        "metric": "ridge readout seconds, FLIF-GL N=4096 T=20000 Dout=100",
+        "value": t, "unit": "s", "n_gpus": 1, "higher_is_better": False, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "ridge readout of the FLIF-GL N=4096 L=512 Watts-Strogatz k=20 reservoir: "
+                               "100 delayed-recall targets, all 20000 recorded states, lambda 1e-5",
+                   "state_record_bytes": n * K * 8},
+        "fit_seconds_all": times,
+        "solver": "LU" if flags & 1 else "Cholesky",
+        "relative_residual": resid,
+        "w_out_finite": bool(np.isfinite(W).all()),
+        "roofline": {"bound": "mfma", "achieved": flop / t / 1e12, "peak": 78.6, "unit": "TFLOP/s",
+                     "frac": flop / t / 1e12 / 78.6, "traffic": None,
+                     "kernel": "rocBLAS dsyrk/dgemm + rocSOLVER dpotrf/dpotrs (whole fit)"},
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -455,6 +502,8 @@ def main():
     ap.add_argument("--fast", action="store_true",
                     help="run the fast-mode comparison (FLIF-GL N=16384, L=1024) instead of the step benchmark")
     ap.add_argument("--fast-steps", type=int, default=50000)
+    ap.add_argument("--readout", action="store_true",
+                    help="time the ridge readout of the N=4096 workload (100 delayed-recall outputs)")
     ap.add_argument("--batched", action="store_true",
                     help="run the batched workload (64 uneven reservoirs) instead of the step benchmark")
     args = ap.parse_args()
@@ -477,6 +526,9 @@ def main():
         return
     if args.batched:
         run_batched(args)
+        return
+    if args.readout:
+        run_readout(args)
         return
     if args.copy_calibration:
         import torch
