@@ -326,7 +326,8 @@ def _levy_steps(rng, count, dim, mu=1.3, lo=1e-3, hi=0.3):
 
 def surrogate_search(ev, rungs, target=None, budget=512, warm=96, wave=32, promote=4, pool=2048, seed=1,
                      epochs=3000, round_epochs=500, lr=1e-3, reseed=6, reseed_top=3, reseed_direct=True,
-                     explore=8, top_weight=4, clip=2.0, novelty=0.5, levy_hi=0.15, log=None):
+                     explore=8, top_weight=4, clip=2.0, novelty=0.5, levy_hi=0.15, low_rungs_warm_only=True,
+                     log=None):
     """Surrogate-guided search with the fidelity rungs.
 
     Every trial first runs to the middle rung (the lowest rung is a checkpoint
@@ -348,6 +349,9 @@ def surrogate_search(ev, rungs, target=None, budget=512, warm=96, wave=32, promo
         by predicted score minus `novelty` x distance to the nearest trial run;
       - the rest from its ranking of `pool` Levy-flight moves around the eight
         best top-rung trials.
+    With `low_rungs_warm_only`, rungs below the middle one are scored for the
+    first wave only (they tell the surrogate how scores change with fidelity);
+    later waves run straight to the middle rung.
     Every reported score comes from a full run; the search stops when a
     top-rung score is within 2 % of target or after `budget` trials."""
     t0 = time.perf_counter()
@@ -383,15 +387,16 @@ def surrogate_search(ev, rungs, target=None, budget=512, warm=96, wave=32, promo
     def rank(v):
         return np.argsort(np.argsort(v))
 
-    def run_wave(new, ep, forced=0):
+    def run_wave(new, ep, forced=0, fids=None):
         nonlocal best
         # the first `forced` trials go to the top rung whatever their middle-rung score
+        fids = lower if fids is None else fids
         for a in range(0, len(new), 32):
             part = new[a:a + 32]
             ev.prefetch(new[a + 32:a + 64])
-            for F in lower:
+            for F in fids:
                 ev.advance(part, F)
-        record(new, lower)
+        record(new, fids)
         train(ep)
         pred = net.predict(encode(np.array([t.theta for t in new]), f_max, f_min, f_max))
         meas = np.array([t.scores[mid] for t in new])
@@ -434,7 +439,7 @@ def surrogate_search(ev, rungs, target=None, budget=512, warm=96, wave=32, promo
         pl = net.predict(encode(loc, f_max, f_min, f_max))
         pick += [loc[i] for i in np.argsort(pl)[:max(0, wave - len(pick))]]
         run_wave([Trial(th, s) for th, s in zip(pick, seeds(len(pick)))], round_epochs,
-                 forced=reseed if reseed_direct else 0)
+                 forced=reseed if reseed_direct else 0, fids=[mid] if low_rungs_warm_only else None)
         count += len(pick)
         if log:
             log(f"surrogate round {rounds}: {count} trials, best {best:.4f} t {time.perf_counter() - t0:.1f}s "
