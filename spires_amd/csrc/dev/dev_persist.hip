@@ -1394,18 +1394,30 @@ extern "C" int spires_dev_timeblock_create(spires_dev* const* gs, int count, lon
         for (int q = 0; q < r; ++q)
             if (gs[q] == g) { set_error("a reservoir appears twice in a time block"); return SPIRES_DEV_EINVAL; }
     }
-    // pack launches: per neurons-per-workgroup class, greedily up to one workgroup per CU
+    // pack launches per neurons-per-workgroup class, at most one workgroup per CU.
+    // A launch lasts as long as its slowest member, whose step time grows with its
+    // history length: reservoirs are placed longest history first (then most
+    // workgroups first) into the first launch with room, so a shorter history only
+    // fills room next to longer ones and never lengthens a launch.
     std::vector<std::vector<int>> launches;
     for (int nb : {32, 16}) {
-        std::vector<int> cur;
-        int wgs = 0;
-        for (int r = 0; r < count; ++r) {
-            if (plans[r].nb != nb) continue;
-            if (!cur.empty() && wgs + plans[r].G > ncu) { launches.push_back(cur); cur.clear(); wgs = 0; }
-            cur.push_back(r);
-            wgs += plans[r].G;
+        std::vector<int> idx;
+        for (int r = 0; r < count; ++r)
+            if (plans[r].nb == nb) idx.push_back(r);
+        std::stable_sort(idx.begin(), idx.end(), [&](int x, int y) {
+            if (gs[x]->L != gs[y]->L) return gs[x]->L > gs[y]->L;
+            return plans[x].G > plans[y].G;
+        });
+        std::vector<std::vector<int>> bins;
+        std::vector<int> used;
+        for (int r : idx) {
+            size_t b = 0;
+            while (b < bins.size() && used[b] + plans[r].G > ncu) ++b;
+            if (b == bins.size()) { bins.emplace_back(); used.push_back(0); }
+            bins[b].push_back(r);
+            used[b] += plans[r].G;
         }
-        if (!cur.empty()) launches.push_back(cur);
+        for (auto& v : bins) launches.push_back(v);
     }
     spires_dev_timeblock* tb = new spires_dev_timeblock();
     tb->K = K;
