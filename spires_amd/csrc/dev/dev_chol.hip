@@ -1,0 +1,428 @@
+// Batched FP64 ridge systems (the readout fits of a hyperparameter search):
+// many independent normal equations (X_r'X_r + lambda_r I) W_r = X_r'Y_r of one
+// order n, solved together so that every launch spans all systems.
+//
+//  - Gram: lower-triangle tiles of X'X on the FP64 matrix cores
+//    (v_mfma_f64_16x16x4f64, operands straight from the recorded state rows,
+//    which lie n-contiguous per time step), lambda added on the diagonal.
+//  - Cross products X'Y: per-thread row sums over chunks of time steps, the
+//    chunks then added in a fixed order (no atomics: results do not depend on
+//    scheduling).
+//  - Cholesky: right-looking, blocks of 64.  The diagonal block is factored in
+//    LDS and inverted there (forward substitution against I), so the panel
+//    below it is one 64x64x64 product with the inverse; the trailing lower
+//    triangle is updated tile by tile.
+//  - Triangular solves: one workgroup per (system, right-hand side); the
+//    off-diagonal products stream the factor with coalesced column (forward)
+//    or row (backward) segments, the diagonal blocks apply their inverses.
+//
+// Factorisation order and rounding differ from LAPACK's; callers refine the
+// solution against the unfactored matrix.  Reference: the normal equations of
+// reservoir.c:573-679 (ridge readout); the reference solves each output
+// separately with LU.
+#include "dev_internal.hpp"
+
+#include <algorithm>
+#include <cmath>
+
+namespace sdev {
+
+namespace {
+
+constexpr int CB = 64;                 // Cholesky block order
+constexpr int kGramBatch = 16;         // systems per Gram launch (pointers passed by value)
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+
+struct GramArgs {
+    const double* X[kGramBatch];       // state rows of system s: X[t * n + i]
+    double lam[kGramBatch];
+};
+
+__device__ __forceinline__ void tri_decode(int t, int& bi, int& bj) {
+    // lower-triangle index t -> (bi >= bj), row-major over the triangle
+    int b = (int)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
+    while ((b + 1) * (b + 2) / 2 <= t) ++b;
+    while (b * (b + 1) / 2 > t) --b;
+    bi = b;
+    bj = t - b * (b + 1) / 2;
+}
+
+// G = X'X over T rows, lower triangle, TILE x TILE per workgroup (4 waves, each
+// a (TILE/2)^2 quarter).  MFMA operands: A[p][k] = X[t+k][j0+p] and B[k][q] =
+// X[t+k][i0+q] (lane l holds row k = l/16, column l%16 of each), so the result
+// D[p][q] = G[j0+p][i0+q]: register r of lane l holds p = 4r + l/16, q = l%16,
+// and its stores to column-major A (element (i, j) at j*n + i) run along i.
+template <int TILE>
+__global__ __launch_bounds__(256) void k_gram_lower(GramArgs ga, int n, int T, double* A, size_t strideA) {
+    constexpr int WT = TILE / 2, M = WT / 16;
+    const int sys = blockIdx.y;
+    int bi, bj;
+    tri_decode(blockIdx.x, bi, bj);
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int i0 = bi * TILE + (wave >> 1) * WT;
+    const int j0 = bj * TILE + (wave & 1) * WT;
+    if (j0 > i0 + WT - 1) return;      // quarter entirely above the diagonal
+    const int li = lane & 15, lk = lane >> 4;
+    const double* X = ga.X[sys];
+    const double* pa = X + (size_t)lk * n + j0 + li;
+    const double* pb = X + (size_t)lk * n + i0 + li;
+    d4 acc[M][M];
+#pragma unroll
+    for (int m = 0; m < M; ++m)
+#pragma unroll
+        for (int q = 0; q < M; ++q) acc[m][q] = d4{0.0, 0.0, 0.0, 0.0};
+    int t = 0;
+#pragma unroll 2
+    for (; t + 4 <= T; t += 4) {
+        const size_t off = (size_t)t * n;
+        double a[M], b[M];
+#pragma unroll
+        for (int m = 0; m < M; ++m) {
+            a[m] = pa[off + 16 * m];
+            b[m] = pb[off + 16 * m];
+        }
+#pragma unroll
+        for (int m = 0; m < M; ++m)
+#pragma unroll
+            for (int q = 0; q < M; ++q) acc[m][q] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[m], b[q], acc[m][q], 0, 0, 0);
+    }
+    if (t < T) {                       // last 1-3 rows: missing rows contribute zero
+        const bool ok = t + lk < T;
+        const size_t off = ok ? (size_t)t * n : 0;
+        double a[M], b[M];
+#pragma unroll
+        for (int m = 0; m < M; ++m) {
+            a[m] = ok ? pa[off + 16 * m] : 0.0;
+            b[m] = ok ? pb[off + 16 * m] : 0.0;
+        }
+#pragma unroll
+        for (int m = 0; m < M; ++m)
+#pragma unroll
+            for (int q = 0; q < M; ++q) acc[m][q] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[m], b[q], acc[m][q], 0, 0, 0);
+    }
+    double* As = A + (size_t)sys * strideA;
+    const double lam = ga.lam[sys];
+#pragma unroll
+    for (int m = 0; m < M; ++m)
+#pragma unroll
+        for (int q = 0; q < M; ++q)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int i = i0 + 16 * q + li, j = j0 + 16 * m + 4 * r + lk;
+                if (i >= j) As[(size_t)j * n + i] = acc[m][q][r] + (i == j ? lam : 0.0);
+            }
+}
+
+// Partial cross products P[s][c][d][i] = sum over rows t of chunk c of X[t][i] Y[t][d],
+// for outputs d0 .. d0+DP-1.
+constexpr int DP = 8;
+__global__ __launch_bounds__(256) void k_cross_part(GramArgs ga, int n, int T, int D, int d0, int chunk,
+                                                    const double* Y, size_t strideY, double* P, int nch) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    const int c = blockIdx.y, sys = blockIdx.z;
+    if (i >= n) return;
+    const double* X = ga.X[sys];
+    const double* Ys = Y + (size_t)sys * strideY;
+    const int dn = min(DP, D - d0);
+    double acc[DP];
+#pragma unroll
+    for (int d = 0; d < DP; ++d) acc[d] = 0.0;
+    const int t1 = min(T, (c + 1) * chunk);
+    for (int t = c * chunk; t < t1; ++t) {
+        const double x = X[(size_t)t * n + i];
+#pragma unroll
+        for (int d = 0; d < DP; ++d)
+            if (d < dn) acc[d] = fma(x, Ys[(size_t)t * D + d0 + d], acc[d]);
+    }
+    double* Ps = P + ((size_t)sys * nch + c) * DP * n;
+#pragma unroll
+    for (int d = 0; d < DP; ++d)
+        if (d < dn) Ps[(size_t)d * n + i] = acc[d];
+}
+
+// B[s][(d0+d) * n + i] = sum over chunks (in order) of the partials.
+__global__ __launch_bounds__(256) void k_cross_sum(int n, int D, int d0, const double* P, int nch, double* B,
+                                                   size_t strideB) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    const int d = blockIdx.y, sys = blockIdx.z;
+    if (i >= n || d0 + d >= D) return;
+    double s = 0.0;
+    for (int c = 0; c < nch; ++c) s += P[(((size_t)sys * nch + c) * DP + d) * n + i];
+    B[(size_t)sys * strideB + (size_t)(d0 + d) * n + i] = s;
+}
+
+__global__ void k_zero_int(int* p, int count) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count) p[i] = 0;
+}
+
+// Factor the diagonal block kb of each system (lower, in place) and store the
+// inverse of its factor, row-major, in Linv[s][kb].  info[s] records the first
+// non-positive pivot (1-based column) and is never cleared here.
+__global__ __launch_bounds__(256) void k_chol_diag(int n, int kb, double* A, size_t strideA, double* Linv, int* info) {
+    __shared__ double L[CB][CB + 1];
+    __shared__ double Li[CB][CB + 1];
+    __shared__ int bad;
+    const int sys = blockIdx.x, tid = threadIdx.x;
+    double* As = A + (size_t)sys * strideA;
+    const int k = kb * CB;
+    if (tid == 0) bad = 0;
+    for (int e = tid; e < CB * CB; e += 256) {
+        const int c = e / CB, r = e % CB;
+        L[r][c] = r >= c ? As[(size_t)(k + c) * n + k + r] : 0.0;
+        Li[c][r] = 0.0;
+    }
+    __syncthreads();
+    for (int j = 0; j < CB; ++j) {
+        const double d = L[j][j];
+        const bool ok = d > 0.0;                     // false for NaN too
+        const double s = ok ? sqrt(d) : 1.0;
+        __syncthreads();                             // every thread has read L[j][j]
+        if (tid == 0) {
+            L[j][j] = s;
+            if (!ok && bad == 0) bad = k + j + 1;
+        }
+        if (tid > j && tid < CB) L[tid][j] = L[tid][j] / s;
+        __syncthreads();
+        const int m = CB - 1 - j;
+        for (int e = tid; e < m * m; e += 256) {
+            const int r = j + 1 + e / m, c = j + 1 + e % m;
+            if (c <= r) L[r][c] = fma(-L[r][j], L[c][j], L[r][c]);
+        }
+        __syncthreads();
+    }
+    // inverse of the factor, column c by lane c: Li[i][c] = (delta_ic - sum_m<i L[i][m] Li[m][c]) / L[i][i]
+    // (Li[m][c] is zero for m < c, so every lane runs the same loop)
+    if (tid < CB) {
+        const int c = tid;
+        for (int i = 0; i < CB; ++i) {
+            double s = i == c ? 1.0 : 0.0;
+            for (int m = 0; m < i; ++m) s = fma(-L[i][m], Li[m][c], s);
+            Li[i][c] = s / L[i][i];
+        }
+    }
+    __syncthreads();
+    double* Lv = Linv + ((size_t)sys * (n / CB) + kb) * CB * CB;
+    for (int e = tid; e < CB * CB; e += 256) {
+        const int c = e / CB, r = e % CB;
+        if (r >= c) As[(size_t)(k + c) * n + k + r] = L[r][c];
+        Lv[e] = Li[e / CB][e % CB];
+    }
+    if (tid == 0 && bad != 0 && info[sys] == 0) info[sys] = bad;
+}
+
+// Panel below diagonal block kb: L[ib][kb] = A[ib][kb] * Linv_kb^T for ib > kb.
+__global__ __launch_bounds__(256) void k_chol_panel(int n, int kb, double* A, size_t strideA, const double* Linv) {
+    __shared__ double P[CB][CB + 1];
+    __shared__ double V[CB][CB + 1];
+    const int sys = blockIdx.y, tid = threadIdx.x;
+    const int k = kb * CB, i0 = (kb + 1 + blockIdx.x) * CB;
+    double* As = A + (size_t)sys * strideA;
+    const double* Lv = Linv + ((size_t)sys * (n / CB) + kb) * CB * CB;
+    for (int e = tid; e < CB * CB; e += 256) {
+        const int c = e / CB, r = e % CB;
+        P[r][c] = As[(size_t)(k + c) * n + i0 + r];
+        V[c][r] = Lv[e];                             // V[row][col] = Linv[row][col]
+    }
+    __syncthreads();
+    const int tr = tid & 15, tc = tid >> 4;
+    double acc[4][4] = {};
+    for (int m = 0; m < CB; ++m) {
+        double p[4], v[4];
+#pragma unroll
+        for (int a = 0; a < 4; ++a) p[a] = P[tr + 16 * a][m];
+#pragma unroll
+        for (int b = 0; b < 4; ++b) v[b] = V[tc + 16 * b][m];
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+            for (int b = 0; b < 4; ++b) acc[a][b] = fma(p[a], v[b], acc[a][b]);
+    }
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) As[(size_t)(k + tc + 16 * b) * n + i0 + tr + 16 * a] = acc[a][b];
+}
+
+// Trailing update after block kb: A[ib][jb] -= L[ib][kb] L[jb][kb]^T for kb < jb <= ib.
+__global__ __launch_bounds__(256) void k_chol_update(int n, int kb, double* A, size_t strideA) {
+    __shared__ double Pi[CB][CB + 1];
+    __shared__ double Pj[CB][CB + 1];
+    const int sys = blockIdx.y, tid = threadIdx.x;
+    int bi, bj;
+    tri_decode(blockIdx.x, bi, bj);
+    const int k = kb * CB, i0 = (kb + 1 + bi) * CB, j0 = (kb + 1 + bj) * CB;
+    double* As = A + (size_t)sys * strideA;
+    for (int e = tid; e < CB * CB; e += 256) {
+        const int c = e / CB, r = e % CB;
+        Pi[r][c] = As[(size_t)(k + c) * n + i0 + r];
+        Pj[r][c] = As[(size_t)(k + c) * n + j0 + r];
+    }
+    __syncthreads();
+    const int tr = tid & 15, tc = tid >> 4;
+    double acc[4][4] = {};
+    for (int m = 0; m < CB; ++m) {
+        double p[4], v[4];
+#pragma unroll
+        for (int a = 0; a < 4; ++a) p[a] = Pi[tr + 16 * a][m];
+#pragma unroll
+        for (int b = 0; b < 4; ++b) v[b] = Pj[tc + 16 * b][m];
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+            for (int b = 0; b < 4; ++b) acc[a][b] = fma(p[a], v[b], acc[a][b]);
+    }
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            const int r = tr + 16 * a, c = tc + 16 * b;
+            if (i0 != j0 || r >= c) {
+                double* p = As + (size_t)(j0 + c) * n + i0 + r;
+                *p = *p - acc[a][b];
+            }
+        }
+}
+
+constexpr int kSolveThreads = 1024;
+
+// x <- L^-1 x for column d = blockIdx.y of system blockIdx.x.
+__global__ __launch_bounds__(kSolveThreads) void k_chol_fwd(int n, const double* A, size_t strideA,
+                                                            const double* Linv, double* Xv, size_t strideX) {
+    extern __shared__ double z[];
+    __shared__ double part[kSolveThreads / CB][CB];
+    __shared__ double rb[CB];
+    const int sys = blockIdx.x, tid = threadIdx.x;
+    const int nbk = n / CB;
+    const double* As = A + (size_t)sys * strideA;
+    double* x = Xv + (size_t)sys * strideX + (size_t)blockIdx.y * n;
+    constexpr int PH = kSolveThreads / CB;
+    const int r = tid % CB, ph = tid / CB;
+    for (int kb = 0; kb < nbk; ++kb) {
+        const int k = kb * CB;
+        double s = 0.0;
+#pragma unroll 4
+        for (int c = ph; c < k; c += PH) s = fma(As[(size_t)c * n + k + r], z[c], s);
+        part[ph][r] = s;
+        __syncthreads();
+        if (tid < CB) {
+            double v = x[k + tid];
+            for (int p = 0; p < PH; ++p) v -= part[p][tid];
+            rb[tid] = v;
+        }
+        __syncthreads();
+        if (tid < CB) {
+            const double* Lv = Linv + ((size_t)sys * nbk + kb) * CB * CB + (size_t)tid * CB;
+            double v = 0.0;
+            for (int m = 0; m <= tid; ++m) v = fma(Lv[m], rb[m], v);
+            z[k + tid] = v;
+        }
+        __syncthreads();
+    }
+    for (int i = tid; i < n; i += kSolveThreads) x[i] = z[i];
+}
+
+// x <- L^-T x for column d = blockIdx.y of system blockIdx.x.
+__global__ __launch_bounds__(kSolveThreads) void k_chol_bwd(int n, const double* A, size_t strideA,
+                                                            const double* Linv, double* Xv, size_t strideX) {
+    extern __shared__ double w[];
+    constexpr int G = kSolveThreads / CB;            // lanes per row of the block
+    __shared__ double part[CB][G + 1];
+    __shared__ double rb[CB];
+    const int sys = blockIdx.x, tid = threadIdx.x;
+    const int nbk = n / CB;
+    const double* As = A + (size_t)sys * strideA;
+    double* x = Xv + (size_t)sys * strideX + (size_t)blockIdx.y * n;
+    const int il = tid / G, g = tid % G;
+    for (int kb = nbk - 1; kb >= 0; --kb) {
+        const int k = kb * CB;
+        // sum over c >= k + CB of L[c][k + il] w[c]; column k+il of L is contiguous in c
+        const double* col = As + (size_t)(k + il) * n;
+        double s = 0.0;
+#pragma unroll 4
+        for (int c = k + CB + g; c < n; c += G) s = fma(col[c], w[c], s);
+        part[il][g] = s;
+        __syncthreads();
+        if (tid < CB) {
+            double v = x[k + tid];
+            for (int p = 0; p < G; ++p) v -= part[tid][p];
+            rb[tid] = v;
+        }
+        __syncthreads();
+        if (tid < CB) {
+            const double* Lv = Linv + ((size_t)sys * nbk + kb) * CB * CB;
+            double v = 0.0;
+            for (int m = tid; m < CB; ++m) v = fma(Lv[(size_t)m * CB + tid], rb[m], v);
+            w[k + tid] = v;
+        }
+        __syncthreads();
+    }
+    for (int i = tid; i < n; i += kSolveThreads) x[i] = w[i];
+}
+
+int cross_chunks(int T) { return std::max(1, std::min(32, (T + 127) / 128)); }
+
+}  // namespace
+
+bool chol_batch_ok(int n) { return n >= CB && n % CB == 0 && n <= kCholMaxN; }
+
+size_t chol_batch_ws_doubles(int count, int n, int T, int D) {
+    (void)D;
+    return (size_t)count * n * CB + (size_t)count * cross_chunks(T) * DP * n;
+}
+
+hipError_t gram_cross_batched(hipStream_t st, int count, int n, int T, int D, const double* const* Xs,
+                              const double* Yd, const double* lam, double* A, size_t strideA, double* B,
+                              size_t strideB, double* ws) {
+    const int nch = cross_chunks(T);
+    const int chunk = (T + nch - 1) / nch;
+    double* P = ws + (size_t)count * n * CB;       // after the Linv blocks
+    for (int s0 = 0; s0 < count; s0 += kGramBatch) {
+        const int cs = std::min(kGramBatch, count - s0);
+        GramArgs ga{};
+        for (int s = 0; s < cs; ++s) {
+            ga.X[s] = Xs[s0 + s];
+            ga.lam[s] = lam[s0 + s];
+        }
+        double* As = A + (size_t)s0 * strideA;
+        if (n % 128 == 0) {
+            const int nt = n / 128;
+            hipLaunchKernelGGL(k_gram_lower<128>, dim3(nt * (nt + 1) / 2, cs), dim3(256), 0, st, ga, n, T, As, strideA);
+        } else {
+            const int nt = n / 64;
+            hipLaunchKernelGGL(k_gram_lower<64>, dim3(nt * (nt + 1) / 2, cs), dim3(256), 0, st, ga, n, T, As, strideA);
+        }
+        const size_t strideY = (size_t)T * D;      // Y of system s at Yd + s * T * D, row-major [T x D]
+        for (int d0 = 0; d0 < D; d0 += DP) {
+            hipLaunchKernelGGL(k_cross_part, dim3((n + 255) / 256, nch, cs), dim3(256), 0, st, ga, n, T, D, d0, chunk,
+                               Yd + (size_t)s0 * strideY, strideY, P, nch);
+            hipLaunchKernelGGL(k_cross_sum, dim3((n + 255) / 256, DP, cs), dim3(256), 0, st, n, D, d0, P, nch,
+                               B + (size_t)s0 * strideB, strideB);
+        }
+    }
+    return hipGetLastError();
+}
+
+hipError_t chol_batched(hipStream_t st, int count, int n, double* A, size_t strideA, double* ws, int* info) {
+    const int nbk = n / CB;
+    hipLaunchKernelGGL(k_zero_int, dim3((count + 255) / 256), dim3(256), 0, st, info, count);
+    for (int kb = 0; kb < nbk; ++kb) {
+        hipLaunchKernelGGL(k_chol_diag, dim3(count), dim3(256), 0, st, n, kb, A, strideA, ws, info);
+        const int rest = nbk - kb - 1;
+        if (rest == 0) break;
+        hipLaunchKernelGGL(k_chol_panel, dim3(rest, count), dim3(256), 0, st, n, kb, A, strideA, ws);
+        hipLaunchKernelGGL(k_chol_update, dim3(rest * (rest + 1) / 2, count), dim3(256), 0, st, n, kb, A, strideA);
+    }
+    return hipGetLastError();
+}
+
+hipError_t chol_solve_batched(hipStream_t st, int count, int n, int D, const double* A, size_t strideA,
+                              const double* ws, double* X, size_t strideX) {
+    const size_t lds = (size_t)n * sizeof(double);
+    hipLaunchKernelGGL(k_chol_fwd, dim3(count, D), dim3(kSolveThreads), lds, st, n, A, strideA, ws, X, strideX);
+    hipLaunchKernelGGL(k_chol_bwd, dim3(count, D), dim3(kSolveThreads), lds, st, n, A, strideA, ws, X, strideX);
+    return hipGetLastError();
+}
+
+}  // namespace sdev

@@ -485,8 +485,12 @@ int spires_dev_ridge_rows_batch(spires_dev* const* gs, int count, size_t row0, s
     const int T = (int)(row1 - row0);
     const size_t nn = (size_t)n * n, nd = (size_t)n * D, td = (size_t)T * D;
     // workspace: A, A0 [count][n x n]; B, X, R [count][n x D]; Y [count][T x D];
-    // info [count] + pivots [n]
-    int rc = lock.reserve(g0, (size_t)count * (2 * nn + 3 * nd + td), (size_t)count + n);
+    // then the batched-factorisation workspace; info [count] + pivots [n].
+    // Orders that are multiples of 64 use the batched Gram / Cholesky / solve
+    // kernels of dev_chol.hip, others rocBLAS + rocSOLVER.
+    const bool own = chol_batch_ok(n);
+    const size_t cw = own ? chol_batch_ws_doubles(count, n, T, D) : 0;
+    int rc = lock.reserve(g0, (size_t)count * (2 * nn + 3 * nd + td) + cw, (size_t)count + n);
     if (rc) return rc;
     double* A = lock.c->ws;
     double* A0 = A + (size_t)count * nn;
@@ -494,23 +498,29 @@ int spires_dev_ridge_rows_batch(spires_dev* const* gs, int count, size_t row0, s
     double* X = B + (size_t)count * nd;
     double* R = X + (size_t)count * nd;
     double* Yd = R + (size_t)count * nd;
+    double* CW = Yd + (size_t)count * td;
     rocblas_int* info = lock.c->iw;
     rocblas_int* ipiv = info + count;
     hipStream_t st = g0->stream;
     const double one = 1.0, zero = 0.0, mone = -1.0;
     rocblas_status sb = rocblas_status_success;
     auto B_ok = [&](rocblas_status x) { if (x != rocblas_status_success && sb == rocblas_status_success) sb = x; };
+    std::vector<const double*> xs(count);
     for (int r = 0; r < count; ++r) {
-        const double* Xs = gs[r]->rec_state + row0 * (size_t)n;   // column-major n x T == X'
+        xs[r] = gs[r]->rec_state + row0 * (size_t)n;   // column-major n x T == X'
         H(hipMemcpyAsync(Yd + r * td, Y[r], td * 8, hipMemcpyHostToDevice, st));
-        B_ok(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_transpose, n, D, T, &one, Xs, n, Yd + r * td, D,
-                           &zero, B + r * nd, n));
-        B_ok(rocblas_dsyrk(h, rocblas_fill_lower, rocblas_operation_none, n, T, &one, Xs, n, &zero, A + r * nn, n));
+        if (own) continue;
+        B_ok(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_transpose, n, D, T, &one, xs[r], n, Yd + r * td,
+                           D, &zero, B + r * nd, n));
+        B_ok(rocblas_dsyrk(h, rocblas_fill_lower, rocblas_operation_none, n, T, &one, xs[r], n, &zero, A + r * nn, n));
         hipLaunchKernelGGL(k_ridge_diag_sym, dim3(nb(nn, 256)), dim3(256), 0, st, n, A + r * nn, lambda[r]);
     }
+    // (own path: lower triangles only; the LU fallback below fills the upper one)
+    if (own) H(gram_cross_batched(st, count, n, T, D, xs.data(), Yd, lambda, A, nn, B, nd, CW));
     H(hipMemcpyAsync(A0, A, (size_t)count * nn * 8, hipMemcpyDeviceToDevice, st));
     H(hipMemcpyAsync(X, B, (size_t)count * nd * 8, hipMemcpyDeviceToDevice, st));
-    B_ok(rocsolver_dpotrf_strided_batched(h, rocblas_fill_lower, n, A, n, (rocblas_stride)nn, info, count));
+    if (own) H(chol_batched(st, count, n, A, nn, CW, info));
+    else B_ok(rocsolver_dpotrf_strided_batched(h, rocblas_fill_lower, n, A, n, (rocblas_stride)nn, info, count));
     std::vector<rocblas_int> hinfo(count, 0);
     H(hipMemcpyAsync(hinfo.data(), info, count * sizeof(rocblas_int), hipMemcpyDeviceToHost, st));
     H(hipStreamSynchronize(st));
@@ -518,13 +528,18 @@ int spires_dev_ridge_rows_batch(spires_dev* const* gs, int count, size_t row0, s
         set_error("batched ridge factorisation failed");
         return SPIRES_DEV_EINTERNAL;
     }
-    B_ok(rocsolver_dpotrs_strided_batched(h, rocblas_fill_lower, n, D, A, n, (rocblas_stride)nn, X, n,
-                                          (rocblas_stride)nd, count));
+    auto solve_all = [&](double* V) {   // V <- A^-1 V with the Cholesky factors
+        if (own) H(chol_solve_batched(st, count, n, D, A, nn, CW, V, nd));
+        else B_ok(rocsolver_dpotrs_strided_batched(h, rocblas_fill_lower, n, D, A, n, (rocblas_stride)nn, V, n,
+                                                   (rocblas_stride)nd, count));
+    };
+    solve_all(X);
     // systems that are not positive definite: LU with partial pivoting, one at a time
     std::vector<char> lu(count, 0), singular(count, 0);
     for (int r = 0; r < count; ++r) {
         if (hinfo[r] == 0) continue;
         lu[r] = 1;
+        if (own) hipLaunchKernelGGL(k_ridge_diag_sym, dim3(nb(nn, 256)), dim3(256), 0, st, n, A0 + r * nn, 0.0);
         H(hipMemcpyAsync(A + r * nn, A0 + r * nn, nn * 8, hipMemcpyDeviceToDevice, st));
         B_ok(rocsolver_dgetrf(h, n, n, A + r * nn, n, ipiv, info));
         rocblas_int i2 = 0;
@@ -548,8 +563,7 @@ int spires_dev_ridge_rows_batch(spires_dev* const* gs, int count, size_t row0, s
         B_ok(rocblas_dsymm_strided_batched(h, rocblas_side_left, rocblas_fill_lower, n, D, &mone, A0, n,
                                            (rocblas_stride)nn, X, n, (rocblas_stride)nd, &one, R, n,
                                            (rocblas_stride)nd, count));
-        B_ok(rocsolver_dpotrs_strided_batched(h, rocblas_fill_lower, n, D, A, n, (rocblas_stride)nn, R, n,
-                                              (rocblas_stride)nd, count));
+        solve_all(R);
         for (int r = 0; r < count; ++r)
             if (!lu[r]) hipLaunchKernelGGL(k_axpy, dim3(nb(nd, 256)), dim3(256), 0, st, nd, R + r * nd, X + r * nd);
     }
