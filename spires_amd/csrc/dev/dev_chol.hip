@@ -4,7 +4,9 @@
 //
 //  - Gram: lower-triangle tiles of X'X on the FP64 matrix cores
 //    (v_mfma_f64_16x16x4f64, operands straight from the recorded state rows,
-//    which lie n-contiguous per time step), lambda added on the diagonal.
+//    which lie n-contiguous per time step), lambda added on the diagonal; each
+//    tile is stored twice (lower and mirrored upper), so the unfactored copy
+//    is a full symmetric matrix for the refinement residual and LU fallback.
 //  - Cross products X'Y: per-thread row sums over chunks of time steps, the
 //    chunks then added in a fixed order (no atomics: results do not depend on
 //    scheduling).
@@ -12,9 +14,12 @@
 //    LDS and inverted there (forward substitution against I), so the panel
 //    below it is one 64x64x64 product with the inverse; the trailing lower
 //    triangle is updated tile by tile.
+//  - The factor kernels also store L' in the (otherwise unused) upper
+//    triangle, so both triangular solves stream coalesced column segments.
 //  - Triangular solves: one workgroup per (system, right-hand side); the
-//    off-diagonal products stream the factor with coalesced column (forward)
-//    or row (backward) segments, the diagonal blocks apply their inverses.
+//    diagonal blocks apply their inverses.
+//  - Residual B - A X of the refinement: a batched GEMV over the full
+//    symmetric copy, column chunks summed in a fixed order.
 //
 // Factorisation order and rounding differ from LAPACK's; callers refine the
 // solution against the unfactored matrix.  Reference: the normal equations of
@@ -54,7 +59,7 @@ __device__ __forceinline__ void tri_decode(int t, int& bi, int& bj) {
 // D[p][q] = G[j0+p][i0+q]: register r of lane l holds p = 4r + l/16, q = l%16,
 // and its stores to column-major A (element (i, j) at j*n + i) run along i.
 template <int TILE>
-__global__ __launch_bounds__(256) void k_gram_lower(GramArgs ga, int n, int T, double* A, size_t strideA) {
+__global__ __launch_bounds__(256, 2) void k_gram_lower(GramArgs ga, int n, int T, double* A, size_t strideA) {
     constexpr int WT = TILE / 2, M = WT / 16;
     const int sys = blockIdx.y;
     int bi, bj;
@@ -73,7 +78,6 @@ __global__ __launch_bounds__(256) void k_gram_lower(GramArgs ga, int n, int T, d
 #pragma unroll
         for (int q = 0; q < M; ++q) acc[m][q] = d4{0.0, 0.0, 0.0, 0.0};
     int t = 0;
-#pragma unroll 2
     for (; t + 4 <= T; t += 4) {
         const size_t off = (size_t)t * n;
         double a[M], b[M];
@@ -103,6 +107,7 @@ __global__ __launch_bounds__(256) void k_gram_lower(GramArgs ga, int n, int T, d
     }
     double* As = A + (size_t)sys * strideA;
     const double lam = ga.lam[sys];
+    const bool diag = i0 == j0;        // a quarter on the diagonal holds both triangles
 #pragma unroll
     for (int m = 0; m < M; ++m)
 #pragma unroll
@@ -110,7 +115,9 @@ __global__ __launch_bounds__(256) void k_gram_lower(GramArgs ga, int n, int T, d
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 const int i = i0 + 16 * q + li, j = j0 + 16 * m + 4 * r + lk;
-                if (i >= j) As[(size_t)j * n + i] = acc[m][q][r] + (i == j ? lam : 0.0);
+                const double v = acc[m][q][r] + (i == j ? lam : 0.0);
+                As[(size_t)j * n + i] = v;
+                if (!diag) As[(size_t)i * n + j] = v;   // mirror (i > j here)
             }
 }
 
@@ -185,28 +192,36 @@ __global__ __launch_bounds__(256) void k_chol_diag(int n, int kb, double* A, siz
         }
         if (tid > j && tid < CB) L[tid][j] = L[tid][j] / s;
         __syncthreads();
-        const int m = CB - 1 - j;
-        for (int e = tid; e < m * m; e += 256) {
-            const int r = j + 1 + e / m, c = j + 1 + e % m;
-            if (c <= r) L[r][c] = fma(-L[r][j], L[c][j], L[r][c]);
+        {   // trailing update: row r = j+1+(tid % 64), columns j+1+(tid / 64) step 4 up to r
+            const int r = j + 1 + (tid & 63);
+            if (r < CB) {
+                const double lr = L[r][j];
+                for (int c = j + 1 + (tid >> 6); c <= r; c += 4) L[r][c] = fma(-lr, L[c][j], L[r][c]);
+            }
         }
         __syncthreads();
     }
-    // inverse of the factor, column c by lane c: Li[i][c] = (delta_ic - sum_m<i L[i][m] Li[m][c]) / L[i][i]
-    // (Li[m][c] is zero for m < c, so every lane runs the same loop)
-    if (tid < CB) {
-        const int c = tid;
+    // inverse of the factor: Li[i][c] = (delta_ic - sum_{m<i} L[i][m] Li[m][c]) / L[i][i],
+    // column c by the four lanes 4c..4c+3 (one wave), lane p summing m = p mod 4; the
+    // sum is completed by shuffles and every lane of the four stores the same value,
+    // so each lane only ever reads LDS entries it wrote itself.  Li[m][c] is zero for
+    // m < c, so all lanes run the same loop.
+    {
+        const int c = tid >> 2, p = tid & 3;
         for (int i = 0; i < CB; ++i) {
-            double s = i == c ? 1.0 : 0.0;
-            for (int m = 0; m < i; ++m) s = fma(-L[i][m], Li[m][c], s);
-            Li[i][c] = s / L[i][i];
+            double s = 0.0;
+            for (int m = p; m < i; m += 4) s = fma(L[i][m], Li[m][c], s);
+            s += __shfl_xor(s, 1);
+            s += __shfl_xor(s, 2);
+            Li[i][c] = ((i == c ? 1.0 : 0.0) - s) / L[i][i];
         }
     }
     __syncthreads();
     double* Lv = Linv + ((size_t)sys * (n / CB) + kb) * CB * CB;
     for (int e = tid; e < CB * CB; e += 256) {
         const int c = e / CB, r = e % CB;
-        if (r >= c) As[(size_t)(k + c) * n + k + r] = L[r][c];
+        if (r >= c) As[(size_t)(k + c) * n + k + r] = L[r][c];        // L (column c, rows r)
+        if (c > r) As[(size_t)(k + c) * n + k + r] = L[c][r];         // L' above the diagonal
         Lv[e] = Li[e / CB][e % CB];
     }
     if (tid == 0 && bad != 0 && info[sys] == 0) info[sys] = bad;
@@ -239,10 +254,19 @@ __global__ __launch_bounds__(256) void k_chol_panel(int n, int kb, double* A, si
 #pragma unroll
             for (int b = 0; b < 4; ++b) acc[a][b] = fma(p[a], v[b], acc[a][b]);
     }
+    __syncthreads();                                 // P is reused for the transposed store
 #pragma unroll
     for (int a = 0; a < 4; ++a)
 #pragma unroll
-        for (int b = 0; b < 4; ++b) As[(size_t)(k + tc + 16 * b) * n + i0 + tr + 16 * a] = acc[a][b];
+        for (int b = 0; b < 4; ++b) {
+            As[(size_t)(k + tc + 16 * b) * n + i0 + tr + 16 * a] = acc[a][b];
+            P[tr + 16 * a][tc + 16 * b] = acc[a][b];
+        }
+    __syncthreads();
+    for (int e = tid; e < CB * CB; e += 256) {      // L' block: row k + c, column i0 + r
+        const int r = e / CB, c = e % CB;
+        As[(size_t)(i0 + r) * n + k + c] = P[r][c];
+    }
 }
 
 // Trailing update after block kb: A[ib][jb] -= L[ib][kb] L[jb][kb]^T for kb < jb <= ib.
@@ -323,30 +347,30 @@ __global__ __launch_bounds__(kSolveThreads) void k_chol_fwd(int n, const double*
     for (int i = tid; i < n; i += kSolveThreads) x[i] = z[i];
 }
 
-// x <- L^-T x for column d = blockIdx.y of system blockIdx.x.
+// x <- L^-T x for column d = blockIdx.y of system blockIdx.x, reading L' from
+// the upper triangle (element (i, c), i < c, at c * n + i).
 __global__ __launch_bounds__(kSolveThreads) void k_chol_bwd(int n, const double* A, size_t strideA,
                                                             const double* Linv, double* Xv, size_t strideX) {
     extern __shared__ double w[];
-    constexpr int G = kSolveThreads / CB;            // lanes per row of the block
-    __shared__ double part[CB][G + 1];
+    constexpr int PH = kSolveThreads / CB;
+    __shared__ double part[PH][CB];
     __shared__ double rb[CB];
     const int sys = blockIdx.x, tid = threadIdx.x;
     const int nbk = n / CB;
     const double* As = A + (size_t)sys * strideA;
     double* x = Xv + (size_t)sys * strideX + (size_t)blockIdx.y * n;
-    const int il = tid / G, g = tid % G;
+    const int r = tid % CB, ph = tid / CB;
     for (int kb = nbk - 1; kb >= 0; --kb) {
         const int k = kb * CB;
-        // sum over c >= k + CB of L[c][k + il] w[c]; column k+il of L is contiguous in c
-        const double* col = As + (size_t)(k + il) * n;
+        // sum over c >= k + CB of L'[k + r][c] w[c]
         double s = 0.0;
 #pragma unroll 4
-        for (int c = k + CB + g; c < n; c += G) s = fma(col[c], w[c], s);
-        part[il][g] = s;
+        for (int c = k + CB + ph; c < n; c += PH) s = fma(As[(size_t)c * n + k + r], w[c], s);
+        part[ph][r] = s;
         __syncthreads();
         if (tid < CB) {
             double v = x[k + tid];
-            for (int p = 0; p < G; ++p) v -= part[tid][p];
+            for (int p = 0; p < PH; ++p) v -= part[p][tid];
             rb[tid] = v;
         }
         __syncthreads();
@@ -361,15 +385,57 @@ __global__ __launch_bounds__(kSolveThreads) void k_chol_bwd(int n, const double*
     for (int i = tid; i < n; i += kSolveThreads) x[i] = w[i];
 }
 
+// Partial products Q[s][ch][d][i] = sum over j in column chunk ch of A[j][i] X[d0 + d][j]
+// (A full symmetric, column-major).
+constexpr int kResChunk = 256;
+__global__ __launch_bounds__(256) void k_resid_part(int n, int D, int d0, const double* A, size_t strideA,
+                                                    const double* X, size_t strideX, double* Q, int nchj) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    const int ch = blockIdx.y, sys = blockIdx.z;
+    if (i >= n) return;
+    const double* As = A + (size_t)sys * strideA;
+    const double* Xs = X + (size_t)sys * strideX;
+    const int dn = min(DP, D - d0);
+    double acc[DP];
+#pragma unroll
+    for (int d = 0; d < DP; ++d) acc[d] = 0.0;
+    const int j1 = min(n, (ch + 1) * kResChunk);
+    for (int j = ch * kResChunk; j < j1; ++j) {
+        const double a = As[(size_t)j * n + i];
+#pragma unroll
+        for (int d = 0; d < DP; ++d)
+            if (d < dn) acc[d] = fma(a, Xs[(size_t)(d0 + d) * n + j], acc[d]);
+    }
+    double* Qs = Q + ((size_t)sys * nchj + ch) * DP * n;
+#pragma unroll
+    for (int d = 0; d < DP; ++d)
+        if (d < dn) Qs[(size_t)d * n + i] = acc[d];
+}
+
+// R = B - sum over chunks (in order) of Q.
+__global__ __launch_bounds__(256) void k_resid_sum(int n, int D, int d0, const double* Q, int nchj, const double* B,
+                                                   double* R, size_t strideB) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    const int d = blockIdx.y, sys = blockIdx.z;
+    if (i >= n || d0 + d >= D) return;
+    double s = 0.0;
+    for (int c = 0; c < nchj; ++c) s += Q[(((size_t)sys * nchj + c) * DP + d) * n + i];
+    const size_t o = (size_t)sys * strideB + (size_t)(d0 + d) * n + i;
+    R[o] = B[o] - s;
+}
+
 int cross_chunks(int T) { return std::max(1, std::min(32, (T + 127) / 128)); }
+int resid_chunks(int n) { return (n + kResChunk - 1) / kResChunk; }
 
 }  // namespace
 
 bool chol_batch_ok(int n) { return n >= CB && n % CB == 0 && n <= kCholMaxN; }
 
+// Workspace: Linv blocks [count][n / 64][64 x 64], residual partials
+// [count][resid_chunks][DP][n], cross-product partials [count][cross_chunks][DP][n].
 size_t chol_batch_ws_doubles(int count, int n, int T, int D) {
     (void)D;
-    return (size_t)count * n * CB + (size_t)count * cross_chunks(T) * DP * n;
+    return (size_t)count * n * CB + (size_t)count * resid_chunks(n) * DP * n + (size_t)count * cross_chunks(T) * DP * n;
 }
 
 hipError_t gram_cross_batched(hipStream_t st, int count, int n, int T, int D, const double* const* Xs,
@@ -377,7 +443,7 @@ hipError_t gram_cross_batched(hipStream_t st, int count, int n, int T, int D, co
                               size_t strideB, double* ws) {
     const int nch = cross_chunks(T);
     const int chunk = (T + nch - 1) / nch;
-    double* P = ws + (size_t)count * n * CB;       // after the Linv blocks
+    double* P = ws + (size_t)count * n * CB + (size_t)count * resid_chunks(n) * DP * n;
     for (int s0 = 0; s0 < count; s0 += kGramBatch) {
         const int cs = std::min(kGramBatch, count - s0);
         GramArgs ga{};
@@ -413,6 +479,19 @@ hipError_t chol_batched(hipStream_t st, int count, int n, double* A, size_t stri
         if (rest == 0) break;
         hipLaunchKernelGGL(k_chol_panel, dim3(rest, count), dim3(256), 0, st, n, kb, A, strideA, ws);
         hipLaunchKernelGGL(k_chol_update, dim3(rest * (rest + 1) / 2, count), dim3(256), 0, st, n, kb, A, strideA);
+    }
+    return hipGetLastError();
+}
+
+hipError_t resid_batched(hipStream_t st, int count, int n, int D, const double* A, size_t strideA, const double* X,
+                         const double* B, double* R, size_t strideX, double* ws) {
+    const int nchj = resid_chunks(n);
+    double* Q = ws + (size_t)count * n * CB;
+    for (int d0 = 0; d0 < D; d0 += DP) {
+        hipLaunchKernelGGL(k_resid_part, dim3((n + 255) / 256, nchj, count), dim3(256), 0, st, n, D, d0, A, strideA, X,
+                           strideX, Q, nchj);
+        hipLaunchKernelGGL(k_resid_sum, dim3((n + 255) / 256, DP, count), dim3(256), 0, st, n, D, d0, Q, nchj, B, R,
+                           strideX);
     }
     return hipGetLastError();
 }

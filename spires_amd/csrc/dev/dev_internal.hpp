@@ -125,8 +125,9 @@ int launch_persist_batch(spires_dev* const* gs, int count, long long t0, long lo
                          uint8_t* const* rec_S);
 int outputs_from_states(spires_dev* g, const double* states, size_t T, double* out);
 // Batched FP64 ridge systems of one order n (dev_chol.hip): A [count][n x n]
-// column-major (lower triangle used), B / X [count][n x D], workspace of
-// chol_batch_ws_doubles() doubles shared by the three calls.
+// column-major (the Gram fills both triangles; the factorisation keeps L in the
+// lower and L' in the upper one), B / X / R [count][n x D], workspace of
+// chol_batch_ws_doubles() doubles shared by the calls.
 constexpr int kCholMaxN = 6144;
 bool chol_batch_ok(int n);
 size_t chol_batch_ws_doubles(int count, int n, int T, int D);
@@ -136,6 +137,9 @@ hipError_t gram_cross_batched(hipStream_t st, int count, int n, int T, int D, co
 hipError_t chol_batched(hipStream_t st, int count, int n, double* A, size_t strideA, double* ws, int* info);
 hipError_t chol_solve_batched(hipStream_t st, int count, int n, int D, const double* A, size_t strideA,
                               const double* ws, double* X, size_t strideX);
+// R = B - A X with A the full symmetric (unfactored) matrices.
+hipError_t resid_batched(hipStream_t st, int count, int n, int D, const double* A, size_t strideA, const double* X,
+                         const double* B, double* R, size_t strideX, double* ws);
 int fast_supported(spires_dev* g);
 int fast_sync_rings(spires_dev* g, bool to_fast);
 int launch_fast(spires_dev* g, const double* u_series, long long t0, long long T, double* rec_V,

@@ -515,7 +515,6 @@ int spires_dev_ridge_rows_batch(spires_dev* const* gs, int count, size_t row0, s
         B_ok(rocblas_dsyrk(h, rocblas_fill_lower, rocblas_operation_none, n, T, &one, xs[r], n, &zero, A + r * nn, n));
         hipLaunchKernelGGL(k_ridge_diag_sym, dim3(nb(nn, 256)), dim3(256), 0, st, n, A + r * nn, lambda[r]);
     }
-    // (own path: lower triangles only; the LU fallback below fills the upper one)
     if (own) H(gram_cross_batched(st, count, n, T, D, xs.data(), Yd, lambda, A, nn, B, nd, CW));
     H(hipMemcpyAsync(A0, A, (size_t)count * nn * 8, hipMemcpyDeviceToDevice, st));
     H(hipMemcpyAsync(X, B, (size_t)count * nd * 8, hipMemcpyDeviceToDevice, st));
@@ -539,7 +538,6 @@ int spires_dev_ridge_rows_batch(spires_dev* const* gs, int count, size_t row0, s
     for (int r = 0; r < count; ++r) {
         if (hinfo[r] == 0) continue;
         lu[r] = 1;
-        if (own) hipLaunchKernelGGL(k_ridge_diag_sym, dim3(nb(nn, 256)), dim3(256), 0, st, n, A0 + r * nn, 0.0);
         H(hipMemcpyAsync(A + r * nn, A0 + r * nn, nn * 8, hipMemcpyDeviceToDevice, st));
         B_ok(rocsolver_dgetrf(h, n, n, A + r * nn, n, ipiv, info));
         rocblas_int i2 = 0;
@@ -557,21 +555,25 @@ int spires_dev_ridge_rows_batch(spires_dev* const* gs, int count, size_t row0, s
             hipLaunchKernelGGL(k_axpy, dim3(nb(nd, 256)), dim3(256), 0, st, nd, R + r * nd, X + r * nd);
         }
     }
-    // iterative refinement of the Cholesky solutions: R = B - A0 X ; solve ; X += R
-    for (int it = 0; it < 2; ++it) {
+    auto resid_all = [&]() {   // R = B - A0 X
+        if (own) {
+            H(resid_batched(st, count, n, D, A0, nn, X, B, R, nd, CW));
+            return;
+        }
         H(hipMemcpyAsync(R, B, (size_t)count * nd * 8, hipMemcpyDeviceToDevice, st));
         B_ok(rocblas_dsymm_strided_batched(h, rocblas_side_left, rocblas_fill_lower, n, D, &mone, A0, n,
                                            (rocblas_stride)nn, X, n, (rocblas_stride)nd, &one, R, n,
                                            (rocblas_stride)nd, count));
+    };
+    // iterative refinement of the Cholesky solutions: R = B - A0 X ; solve ; X += R
+    for (int it = 0; it < 2; ++it) {
+        resid_all();
         solve_all(R);
         for (int r = 0; r < count; ++r)
             if (!lu[r]) hipLaunchKernelGGL(k_axpy, dim3(nb(nd, 256)), dim3(256), 0, st, nd, R + r * nd, X + r * nd);
     }
     if (info_out) {
-        H(hipMemcpyAsync(R, B, (size_t)count * nd * 8, hipMemcpyDeviceToDevice, st));
-        B_ok(rocblas_dsymm_strided_batched(h, rocblas_side_left, rocblas_fill_lower, n, D, &mone, A0, n,
-                                           (rocblas_stride)nn, X, n, (rocblas_stride)nd, &one, R, n,
-                                           (rocblas_stride)nd, count));
+        resid_all();
         std::vector<double> hr((size_t)count * nd), hb((size_t)count * nd);
         H(hipMemcpyAsync(hr.data(), R, hr.size() * 8, hipMemcpyDeviceToHost, st));
         H(hipMemcpyAsync(hb.data(), B, hb.size() * 8, hipMemcpyDeviceToHost, st));
