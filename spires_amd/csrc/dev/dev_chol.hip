@@ -123,7 +123,7 @@ __global__ __launch_bounds__(256, 2) void k_gram_lower(GramArgs ga, int n, int T
 
 // Partial cross products P[s][c][d][i] = sum over rows t of chunk c of X[t][i] Y[t][d],
 // for outputs d0 .. d0+DP-1.
-constexpr int DP = 8;
+constexpr int DP = kCholRhsMax;
 __global__ __launch_bounds__(256) void k_cross_part(GramArgs ga, int n, int T, int D, int d0, int chunk,
                                                     const double* Y, size_t strideY, double* P, int nch) {
     const int i = blockIdx.x * 256 + threadIdx.x;
@@ -440,7 +440,7 @@ size_t chol_batch_ws_doubles(int count, int n, int T, int D) {
 
 hipError_t gram_cross_batched(hipStream_t st, int count, int n, int T, int D, const double* const* Xs,
                               const double* Yd, const double* lam, double* A, size_t strideA, double* B,
-                              size_t strideB, double* ws) {
+                              size_t strideB, double* ws, bool with_cross) {
     const int nch = cross_chunks(T);
     const int chunk = (T + nch - 1) / nch;
     double* P = ws + (size_t)count * n * CB + (size_t)count * resid_chunks(n) * DP * n;
@@ -460,7 +460,7 @@ hipError_t gram_cross_batched(hipStream_t st, int count, int n, int T, int D, co
             hipLaunchKernelGGL(k_gram_lower<64>, dim3(nt * (nt + 1) / 2, cs), dim3(256), 0, st, ga, n, T, As, strideA);
         }
         const size_t strideY = (size_t)T * D;      // Y of system s at Yd + s * T * D, row-major [T x D]
-        for (int d0 = 0; d0 < D; d0 += DP) {
+        for (int d0 = 0; with_cross && d0 < D; d0 += DP) {
             hipLaunchKernelGGL(k_cross_part, dim3((n + 255) / 256, nch, cs), dim3(256), 0, st, ga, n, T, D, d0, chunk,
                                Yd + (size_t)s0 * strideY, strideY, P, nch);
             hipLaunchKernelGGL(k_cross_sum, dim3((n + 255) / 256, DP, cs), dim3(256), 0, st, n, D, d0, P, nch,

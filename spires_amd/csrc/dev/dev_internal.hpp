@@ -129,11 +129,12 @@ int outputs_from_states(spires_dev* g, const double* states, size_t T, double* o
 // lower and L' in the upper one), B / X / R [count][n x D], workspace of
 // chol_batch_ws_doubles() doubles shared by the calls.
 constexpr int kCholMaxN = 6144;
+constexpr int kCholRhsMax = 8;     // right-hand sides handled by the cross / solve / residual kernels
 bool chol_batch_ok(int n);
 size_t chol_batch_ws_doubles(int count, int n, int T, int D);
 hipError_t gram_cross_batched(hipStream_t st, int count, int n, int T, int D, const double* const* Xs,
                               const double* Yd, const double* lam, double* A, size_t strideA, double* B,
-                              size_t strideB, double* ws);
+                              size_t strideB, double* ws, bool with_cross);
 hipError_t chol_batched(hipStream_t st, int count, int n, double* A, size_t strideA, double* ws, int* info);
 hipError_t chol_solve_batched(hipStream_t st, int count, int n, int D, const double* A, size_t strideA,
                               const double* ws, double* X, size_t strideX);

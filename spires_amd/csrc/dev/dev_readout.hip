@@ -259,6 +259,9 @@ using namespace sdev;
 
 extern "C" {
 
+static int ridge_batch_impl(spires_dev* const* gs, int count, size_t row0, size_t row1, const double* const* Y,
+                            const double* lambda, double* info_out);
+
 int spires_dev_ridge(spires_dev* g, size_t washout, const double* Y, double lambda, double* info_out) {
     if (!g) return SPIRES_DEV_EINVAL;
     return spires_dev_ridge_rows(g, washout, g->rec_rows, Y, lambda, info_out);
@@ -274,6 +277,20 @@ int spires_dev_ridge_rows(spires_dev* g, size_t row0, size_t row1, const double*
     const int n = g->n, D = g->dout;
     const int T = (int)(row1 - row0);
     if (D == 0) return 0;
+    if (g->mode == SPIRES_DEV_MODE_EXACT && chol_batch_ok(n)) {   // Gram + Cholesky kernels of dev_chol.hip
+        double inf[2] = {0.0, 0.0};
+        const int rc = ridge_batch_impl(&g, 1, row0, row1, &Y, &lambda, inf);
+        if (rc) return rc;
+        if (info_out) {
+            info_out[0] = inf[0];
+            info_out[1] = inf[1];
+        }
+        if (inf[0] < 0) {   // W_out left unchanged, as the reference does
+            set_error("ridge system is singular");
+            return SPIRES_DEV_EINTERNAL;
+        }
+        return 0;
+    }
     ReadoutLock lock(g);
     rocblas_handle h = lock.h;
     if (!h) { set_error("rocblas_create_handle failed"); return SPIRES_DEV_EINTERNAL; }
@@ -467,13 +484,22 @@ int spires_dev_ridge_rows_batch(spires_dev* const* gs, int count, size_t row0, s
         if (!gs[r]->rec_state || row0 >= row1 || row1 > gs[r]->rec_rows) { set_error("no recorded states for ridge"); return SPIRES_DEV_EINVAL; }
         same = same && gs[r]->n == n && gs[r]->dout == D && gs[r]->mode == SPIRES_DEV_MODE_EXACT;
     }
-    if (!same || count == 1) {   // one system at a time
+    if (!same || (count == 1 && !chol_batch_ok(n))) {   // one system at a time
         for (int r = 0; r < count; ++r) {
             int rc = spires_dev_ridge_rows(gs[r], row0, row1, Y[r], lambda[r], info_out ? info_out + 2 * r : nullptr);
             if (rc) return rc;
         }
         return 0;
     }
+    return ridge_batch_impl(gs, count, row0, row1, Y, lambda, info_out);
+}
+
+// Exact-mode systems of one order n and output count D, factorised together.
+// Singular systems (LU fails too) keep their W_out and report -1.
+static int ridge_batch_impl(spires_dev* const* gs, int count, size_t row0, size_t row1, const double* const* Y,
+                            const double* lambda, double* info_out) {
+    spires_dev* g0 = gs[0];
+    const int n = g0->n, D = g0->dout;
     if (D == 0) return 0;
     hipError_t herr = hipSuccess;
     auto H = [&](hipError_t e) { if (e != hipSuccess && herr == hipSuccess) herr = e; };
@@ -489,6 +515,7 @@ int spires_dev_ridge_rows_batch(spires_dev* const* gs, int count, size_t row0, s
     // Orders that are multiples of 64 use the batched Gram / Cholesky / solve
     // kernels of dev_chol.hip, others rocBLAS + rocSOLVER.
     const bool own = chol_batch_ok(n);
+    const bool own_rhs = own && D <= kCholRhsMax;   // cross products, solves, residuals too
     const size_t cw = own ? chol_batch_ws_doubles(count, n, T, D) : 0;
     int rc = lock.reserve(g0, (size_t)count * (2 * nn + 3 * nd + td) + cw, (size_t)count + n);
     if (rc) return rc;
@@ -509,13 +536,14 @@ int spires_dev_ridge_rows_batch(spires_dev* const* gs, int count, size_t row0, s
     for (int r = 0; r < count; ++r) {
         xs[r] = gs[r]->rec_state + row0 * (size_t)n;   // column-major n x T == X'
         H(hipMemcpyAsync(Yd + r * td, Y[r], td * 8, hipMemcpyHostToDevice, st));
-        if (own) continue;
+        if (own_rhs) continue;
         B_ok(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_transpose, n, D, T, &one, xs[r], n, Yd + r * td,
                            D, &zero, B + r * nd, n));
+        if (own) continue;
         B_ok(rocblas_dsyrk(h, rocblas_fill_lower, rocblas_operation_none, n, T, &one, xs[r], n, &zero, A + r * nn, n));
         hipLaunchKernelGGL(k_ridge_diag_sym, dim3(nb(nn, 256)), dim3(256), 0, st, n, A + r * nn, lambda[r]);
     }
-    if (own) H(gram_cross_batched(st, count, n, T, D, xs.data(), Yd, lambda, A, nn, B, nd, CW));
+    if (own) H(gram_cross_batched(st, count, n, T, D, xs.data(), Yd, lambda, A, nn, B, nd, CW, own_rhs));
     H(hipMemcpyAsync(A0, A, (size_t)count * nn * 8, hipMemcpyDeviceToDevice, st));
     H(hipMemcpyAsync(X, B, (size_t)count * nd * 8, hipMemcpyDeviceToDevice, st));
     if (own) H(chol_batched(st, count, n, A, nn, CW, info));
@@ -528,9 +556,17 @@ int spires_dev_ridge_rows_batch(spires_dev* const* gs, int count, size_t row0, s
         return SPIRES_DEV_EINTERNAL;
     }
     auto solve_all = [&](double* V) {   // V <- A^-1 V with the Cholesky factors
-        if (own) H(chol_solve_batched(st, count, n, D, A, nn, CW, V, nd));
-        else B_ok(rocsolver_dpotrs_strided_batched(h, rocblas_fill_lower, n, D, A, n, (rocblas_stride)nn, V, n,
-                                                   (rocblas_stride)nd, count));
+        if (own_rhs) {
+            H(chol_solve_batched(st, count, n, D, A, nn, CW, V, nd));
+            return;
+        }
+        // L Z = V, then L' X = Z (BLAS3 triangular solves over all D columns)
+        B_ok(rocblas_dtrsm_strided_batched(h, rocblas_side_left, rocblas_fill_lower, rocblas_operation_none,
+                                           rocblas_diagonal_non_unit, n, D, &one, A, n, (rocblas_stride)nn, V, n,
+                                           (rocblas_stride)nd, count));
+        B_ok(rocblas_dtrsm_strided_batched(h, rocblas_side_left, rocblas_fill_lower, rocblas_operation_transpose,
+                                           rocblas_diagonal_non_unit, n, D, &one, A, n, (rocblas_stride)nn, V, n,
+                                           (rocblas_stride)nd, count));
     };
     solve_all(X);
     // systems that are not positive definite: LU with partial pivoting, one at a time
@@ -556,7 +592,7 @@ int spires_dev_ridge_rows_batch(spires_dev* const* gs, int count, size_t row0, s
         }
     }
     auto resid_all = [&]() {   // R = B - A0 X
-        if (own) {
+        if (own_rhs) {
             H(resid_batched(st, count, n, D, A0, nn, X, B, R, nd, CW));
             return;
         }
