@@ -446,9 +446,11 @@ def run_readout(args):
     delayed-recall targets u[t-d], d=1..100, fitted on all T=20000 recorded
     states with lambda=1e-5 (train_output_ridge_regression records every step,
     reservoir.c:573-679).  A = X'X + lambda I and B = X'Y are formed on the
-    device from the state record in HBM (655 MB), then one FP64 Cholesky for all
-    100 outputs and two refinement steps.  Reports the fit time, its FP64 rate
-    against the MFMA peak and the relative normal-equation residual."""
+    device from the state record in HBM (655 MB) (the Gram by the FP64
+    matrix-core kernel), then one FP64 Cholesky (blocked kernels) for all 100
+    outputs, batched triangular solves and two refinement steps.  Reports the
+    fit time, its FP64 rate against the MFMA peak and the relative
+    normal-equation residual."""
     import numpy as np
     import spires_amd as sa
     from spires_amd import workloads
@@ -483,7 +485,7 @@ def run_readout(args):
         "w_out_finite": bool(np.isfinite(W).all()),
         "roofline": {"bound": "mfma", "achieved": flop / t / 1e12, "peak": 78.6, "unit": "TFLOP/s",
                      "frac": flop / t / 1e12 / 78.6, "traffic": None,
-                     "kernel": "rocBLAS dsyrk/dgemm + rocSOLVER dpotrf/dpotrs (whole fit)"},
+                     "kernel": "k_gram_lower + k_chol_* + rocBLAS dgemm/dtrsm (whole fit)"},
     }
     print(json.dumps(line), flush=True)
 
