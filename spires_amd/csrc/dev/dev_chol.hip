@@ -385,6 +385,86 @@ __global__ __launch_bounds__(kSolveThreads) void k_chol_bwd(int n, const double*
     for (int i = tid; i < n; i += kSolveThreads) x[i] = w[i];
 }
 
+// Blocked triangular solves for many right-hand sides.  k_trsm_diag: rows of block kb,
+// 64 columns per workgroup, X <- M X with M = Linv_kb (forward) or Linv_kb' (backward).
+__global__ __launch_bounds__(256) void k_trsm_diag(int n, int D, int kb, int trans, const double* Linv, double* X,
+                                                   size_t strideX) {
+    __shared__ double Bs[CB][CB + 1];                // Bs[m][c] = X(k + m, d0 + c)
+    __shared__ double V[CB][CB + 1];                 // V[i][m] = M[i][m]
+    const int d0 = blockIdx.x * CB, sys = blockIdx.y, tid = threadIdx.x;
+    const int nbk = n / CB, k = kb * CB;
+    const double* Lv = Linv + ((size_t)sys * nbk + kb) * CB * CB;
+    double* Xs = X + (size_t)sys * strideX;
+    for (int e = tid; e < CB * CB; e += 256) {
+        const int c = e / CB, m = e % CB;
+        Bs[m][c] = d0 + c < D ? Xs[(size_t)(d0 + c) * n + k + m] : 0.0;
+        V[c][m] = trans ? Lv[(size_t)m * CB + c] : Lv[e];
+    }
+    __syncthreads();
+    const int tr = tid & 15, tc = tid >> 4;
+    double acc[4][4] = {};
+    for (int m = 0; m < CB; ++m) {
+        double p[4], v[4];
+#pragma unroll
+        for (int a = 0; a < 4; ++a) p[a] = V[tr + 16 * a][m];
+#pragma unroll
+        for (int b = 0; b < 4; ++b) v[b] = Bs[m][tc + 16 * b];
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+            for (int b = 0; b < 4; ++b) acc[a][b] = fma(p[a], v[b], acc[a][b]);
+    }
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+        const int c = tc + 16 * b;
+        if (d0 + c < D)
+#pragma unroll
+            for (int a = 0; a < 4; ++a) Xs[(size_t)(d0 + c) * n + k + tr + 16 * a] = acc[a][b];
+    }
+}
+
+// X rows of block ib = ib0 + blockIdx.x minus A-block (ib, kb) times X rows of block kb,
+// 64 columns per workgroup.  The A-block is L[ib][kb] below the diagonal (forward) and
+// L'[ib][kb] above it (backward); both sit at column-major (kb*64 + m) * n + ib*64 + r.
+__global__ __launch_bounds__(256) void k_trsm_update(int n, int D, int kb, int ib0, const double* A, size_t strideA,
+                                                     double* X, size_t strideX) {
+    __shared__ double P[CB][CB + 1];                 // P[r][m] = A(ib*64 + r, kb*64 + m)
+    __shared__ double Bs[CB][CB + 1];                // Bs[m][c] = X(kb*64 + m, d0 + c)
+    const int ib = ib0 + blockIdx.x, d0 = blockIdx.y * CB, sys = blockIdx.z, tid = threadIdx.x;
+    const int k = kb * CB, i0 = ib * CB;
+    const double* As = A + (size_t)sys * strideA;
+    double* Xs = X + (size_t)sys * strideX;
+    for (int e = tid; e < CB * CB; e += 256) {
+        const int c = e / CB, r = e % CB;
+        P[r][c] = As[(size_t)(k + c) * n + i0 + r];
+        Bs[r][c] = d0 + c < D ? Xs[(size_t)(d0 + c) * n + k + r] : 0.0;
+    }
+    __syncthreads();
+    const int tr = tid & 15, tc = tid >> 4;
+    double acc[4][4] = {};
+    for (int m = 0; m < CB; ++m) {
+        double p[4], v[4];
+#pragma unroll
+        for (int a = 0; a < 4; ++a) p[a] = P[tr + 16 * a][m];
+#pragma unroll
+        for (int b = 0; b < 4; ++b) v[b] = Bs[m][tc + 16 * b];
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+            for (int b = 0; b < 4; ++b) acc[a][b] = fma(p[a], v[b], acc[a][b]);
+    }
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+        const int c = tc + 16 * b;
+        if (d0 + c < D)
+#pragma unroll
+            for (int a = 0; a < 4; ++a) {
+                double* q = Xs + (size_t)(d0 + c) * n + i0 + tr + 16 * a;
+                *q = *q - acc[a][b];
+            }
+    }
+}
+
 // Partial products Q[s][ch][d][i] = sum over j in column chunk ch of A[j][i] X[d0 + d][j]
 // (A full symmetric, column-major).
 constexpr int kResChunk = 256;
@@ -492,6 +572,24 @@ hipError_t resid_batched(hipStream_t st, int count, int n, int D, const double* 
                            strideX, Q, nchj);
         hipLaunchKernelGGL(k_resid_sum, dim3((n + 255) / 256, DP, count), dim3(256), 0, st, n, D, d0, Q, nchj, B, R,
                            strideX);
+    }
+    return hipGetLastError();
+}
+
+hipError_t chol_solve_many(hipStream_t st, int count, int n, int D, const double* A, size_t strideA,
+                           const double* ws, double* X, size_t strideX) {
+    const int nbk = n / CB, dc = (D + CB - 1) / CB;
+    for (int kb = 0; kb < nbk; ++kb) {               // L Z = X
+        hipLaunchKernelGGL(k_trsm_diag, dim3(dc, count), dim3(256), 0, st, n, D, kb, 0, ws, X, strideX);
+        if (kb + 1 < nbk)
+            hipLaunchKernelGGL(k_trsm_update, dim3(nbk - kb - 1, dc, count), dim3(256), 0, st, n, D, kb, kb + 1, A,
+                               strideA, X, strideX);
+    }
+    for (int kb = nbk - 1; kb >= 0; --kb) {          // L' X = Z
+        hipLaunchKernelGGL(k_trsm_diag, dim3(dc, count), dim3(256), 0, st, n, D, kb, 1, ws, X, strideX);
+        if (kb > 0)
+            hipLaunchKernelGGL(k_trsm_update, dim3(kb, dc, count), dim3(256), 0, st, n, D, kb, 0, A, strideA, X,
+                               strideX);
     }
     return hipGetLastError();
 }

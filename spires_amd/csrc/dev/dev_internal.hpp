@@ -138,6 +138,10 @@ hipError_t gram_cross_batched(hipStream_t st, int count, int n, int T, int D, co
 hipError_t chol_batched(hipStream_t st, int count, int n, double* A, size_t strideA, double* ws, int* info);
 hipError_t chol_solve_batched(hipStream_t st, int count, int n, int D, const double* A, size_t strideA,
                               const double* ws, double* X, size_t strideX);
+// The same solve for any number of right-hand sides: blocked by 64 rows x 64 columns,
+// two launches per block and direction.
+hipError_t chol_solve_many(hipStream_t st, int count, int n, int D, const double* A, size_t strideA,
+                           const double* ws, double* X, size_t strideX);
 // R = B - A X with A the full symmetric (unfactored) matrices.
 hipError_t resid_batched(hipStream_t st, int count, int n, int D, const double* A, size_t strideA, const double* X,
                          const double* B, double* R, size_t strideX, double* ws);

@@ -560,6 +560,10 @@ static int ridge_batch_impl(spires_dev* const* gs, int count, size_t row0, size_
             H(chol_solve_batched(st, count, n, D, A, nn, CW, V, nd));
             return;
         }
+        if (own) {                  // the kernels' factor: blocked solves over all D columns
+            H(chol_solve_many(st, count, n, D, A, nn, CW, V, nd));
+            return;
+        }
         // L Z = V, then L' X = Z (BLAS3 triangular solves over all D columns)
         B_ok(rocblas_dtrsm_strided_batched(h, rocblas_side_left, rocblas_fill_lower, rocblas_operation_none,
                                            rocblas_diagonal_non_unit, n, D, &one, A, n, (rocblas_stride)nn, V, n,

@@ -93,3 +93,17 @@ def test_order_not_multiple_of_64_keeps_library_path():
         kappa = np.linalg.cond(A)
         assert np.abs(r.W_out() - W).max() <= max(1e-9, 1e-14 * kappa) * np.abs(W).max()
         r.close()
+
+
+@pytest.mark.parametrize("n,count", [(64, 2), (128, 18)])
+def test_batched_fit_edge_orders_and_counts(n, count):
+    """One 64x64 block (no panel or trailing update), and more systems than one
+    Gram launch takes (16); 553 rows leave a partial group of 4 for the Gram."""
+    T, r0, r1 = 700, 50, 603
+    rs, y = _runs(n, count, T, seed=43, dout=1)
+    lams = [10.0 ** -(2 + k % 5) for k in range(count)]
+    info = sa.fit_ridge_records_batch(rs, r0, r1, [y[r0:r1]] * count, lams)
+    assert np.all(info[:, 0] == 0) and np.all(info[:, 1] < 1e-10)
+    for r, lam in zip(rs, lams):
+        _numpy_check(r, r0, r1, y, lam, r.W_out())
+        r.close()
