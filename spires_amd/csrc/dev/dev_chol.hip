@@ -10,9 +10,9 @@
 //  - Cross products X'Y: per-thread row sums over chunks of time steps, the
 //    chunks then added in a fixed order (no atomics: results do not depend on
 //    scheduling).
-//  - Cholesky: right-looking, blocks of 64.  The diagonal block is factored in
-//    LDS and inverted there (forward substitution against I), so the panel
-//    below it is one 64x64x64 product with the inverse; the trailing lower
+//  - Cholesky: right-looking, blocks of 64.  The diagonal block is factored by
+//    one wave in registers and inverted (forward substitution against I), so the
+//    panel below it is one 64x64x64 product with the inverse; the trailing lower
 //    triangle is updated tile by tile.
 //  - The factor kernels also store L' in the (otherwise unused) upper
 //    triangle, so both triangular solves stream coalesced column segments.
@@ -164,67 +164,59 @@ __global__ void k_zero_int(int* p, int count) {
     if (i < count) p[i] = 0;
 }
 
-// Factor the diagonal block kb of each system (lower, in place) and store the
-// inverse of its factor, row-major, in Linv[s][kb].  info[s] records the first
+__device__ __forceinline__ double lane_bcast(double x, int l) {   // x of lane l (l wave-uniform)
+    const unsigned long long b = (unsigned long long)__double_as_longlong(x);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)b, l);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), l);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+// Factor the diagonal block kb of each system (lower, in place; L' into the upper half
+// of the block) and store the inverse of its factor, row-major, in Linv[s][kb].  One
+// wave: lane r keeps row r of the block in registers and takes column entries of other
+// rows by readlane, so the 64 column steps need no barrier.  Entries above the
+// diagonal are updated too but never stored, and never feed an entry below it.  The
+// inverse: lane c computes column c, Li[i][c] = (delta_ic - sum_{m<i} L[i][m] Li[m][c])
+// / L[i][i], with the factor's rows broadcast from LDS.  info[s] records the first
 // non-positive pivot (1-based column) and is never cleared here.
-__global__ __launch_bounds__(256) void k_chol_diag(int n, int kb, double* A, size_t strideA, double* Linv, int* info) {
+__global__ __launch_bounds__(CB) void k_chol_diag(int n, int kb, double* A, size_t strideA, double* Linv, int* info) {
     __shared__ double L[CB][CB + 1];
-    __shared__ double Li[CB][CB + 1];
-    __shared__ int bad;
-    const int sys = blockIdx.x, tid = threadIdx.x;
+    const int sys = blockIdx.x, lane = threadIdx.x;
     double* As = A + (size_t)sys * strideA;
     const int k = kb * CB;
-    if (tid == 0) bad = 0;
-    for (int e = tid; e < CB * CB; e += 256) {
-        const int c = e / CB, r = e % CB;
-        L[r][c] = r >= c ? As[(size_t)(k + c) * n + k + r] : 0.0;
-        Li[c][r] = 0.0;
-    }
-    __syncthreads();
+    double row[CB];
+#pragma unroll
+    for (int c = 0; c < CB; ++c) row[c] = As[(size_t)(k + c) * n + k + lane];
+    int bad = 0;
+#pragma unroll
     for (int j = 0; j < CB; ++j) {
-        const double d = L[j][j];
+        const double d = lane_bcast(row[j], j);
         const bool ok = d > 0.0;                     // false for NaN too
         const double s = ok ? sqrt(d) : 1.0;
-        __syncthreads();                             // every thread has read L[j][j]
-        if (tid == 0) {
-            L[j][j] = s;
-            if (!ok && bad == 0) bad = k + j + 1;
-        }
-        if (tid > j && tid < CB) L[tid][j] = L[tid][j] / s;
-        __syncthreads();
-        {   // trailing update: row r = j+1+(tid % 64), columns j+1+(tid / 64) step 4 up to r
-            const int r = j + 1 + (tid & 63);
-            if (r < CB) {
-                const double lr = L[r][j];
-                for (int c = j + 1 + (tid >> 6); c <= r; c += 4) L[r][c] = fma(-lr, L[c][j], L[r][c]);
-            }
-        }
-        __syncthreads();
+        if (!ok && bad == 0) bad = k + j + 1;
+        row[j] = lane == j ? s : row[j] / s;
+#pragma unroll
+        for (int c = j + 1; c < CB; ++c) row[c] = fma(-row[j], lane_bcast(row[j], c), row[c]);
     }
-    // inverse of the factor: Li[i][c] = (delta_ic - sum_{m<i} L[i][m] Li[m][c]) / L[i][i],
-    // column c by the four lanes 4c..4c+3 (one wave), lane p summing m = p mod 4; the
-    // sum is completed by shuffles and every lane of the four stores the same value,
-    // so each lane only ever reads LDS entries it wrote itself.  Li[m][c] is zero for
-    // m < c, so all lanes run the same loop.
-    {
-        const int c = tid >> 2, p = tid & 3;
-        for (int i = 0; i < CB; ++i) {
-            double s = 0.0;
-            for (int m = p; m < i; m += 4) s = fma(L[i][m], Li[m][c], s);
-            s += __shfl_xor(s, 1);
-            s += __shfl_xor(s, 2);
-            Li[i][c] = ((i == c ? 1.0 : 0.0) - s) / L[i][i];
-        }
-    }
+#pragma unroll
+    for (int c = 0; c < CB; ++c) L[lane][c] = c <= lane ? row[c] : 0.0;
     __syncthreads();
-    double* Lv = Linv + ((size_t)sys * (n / CB) + kb) * CB * CB;
-    for (int e = tid; e < CB * CB; e += 256) {
+    for (int e = lane; e < CB * CB; e += CB) {
         const int c = e / CB, r = e % CB;
-        if (r >= c) As[(size_t)(k + c) * n + k + r] = L[r][c];        // L (column c, rows r)
-        if (c > r) As[(size_t)(k + c) * n + k + r] = L[c][r];         // L' above the diagonal
-        Lv[e] = Li[e / CB][e % CB];
+        As[(size_t)(k + c) * n + k + r] = r >= c ? L[r][c] : L[c][r];   // L below, L' above
     }
-    if (tid == 0 && bad != 0 && info[sys] == 0) info[sys] = bad;
+    double li[CB];
+#pragma unroll
+    for (int i = 0; i < CB; ++i) {
+        double s = i == lane ? 1.0 : 0.0;
+#pragma unroll
+        for (int m = 0; m < i; ++m) s = fma(-L[i][m], li[m], s);
+        li[i] = s / L[i][i];
+    }
+    double* Lv = Linv + ((size_t)sys * (n / CB) + kb) * CB * CB;
+#pragma unroll
+    for (int i = 0; i < CB; ++i) Lv[(size_t)i * CB + lane] = li[i];
+    if (lane == 0 && bad != 0 && info[sys] == 0) info[sys] = bad;
 }
 
 // Panel below diagonal block kb: L[ib][kb] = A[ib][kb] * Linv_kb^T for ib > kb.
@@ -554,7 +546,7 @@ hipError_t chol_batched(hipStream_t st, int count, int n, double* A, size_t stri
     const int nbk = n / CB;
     hipLaunchKernelGGL(k_zero_int, dim3((count + 255) / 256), dim3(256), 0, st, info, count);
     for (int kb = 0; kb < nbk; ++kb) {
-        hipLaunchKernelGGL(k_chol_diag, dim3(count), dim3(256), 0, st, n, kb, A, strideA, ws, info);
+        hipLaunchKernelGGL(k_chol_diag, dim3(count), dim3(CB), 0, st, n, kb, A, strideA, ws, info);
         const int rest = nbk - kb - 1;
         if (rest == 0) break;
         hipLaunchKernelGGL(k_chol_panel, dim3(rest, count), dim3(256), 0, st, n, kb, A, strideA, ws);
