@@ -110,13 +110,17 @@ class Evaluator:
     them to a fidelity on the GPU: batched runs, batched ridge fits, readout
     predictions on the device."""
 
-    def __init__(self, n=2048, f_max=18000, seed=0, workers=4):
+    def __init__(self, n=2048, f_max=18000, seed=0, workers=8):
         self.n = n
         self.f_max = f_max
         series = workloads.mackey_glass(f_max + HORIZON + 1, seed=seed)
         self.u = np.ascontiguousarray(series[:f_max].reshape(f_max, 1))
         self.y = np.ascontiguousarray(series[HORIZON:f_max + HORIZON])
         self.pool = ThreadPoolExecutor(max_workers=workers)
+        # released trials are destroyed on a thread of their own (their frees then
+        # overlap the next launches); drain() waits for them
+        self.closer = ThreadPoolExecutor(max_workers=1)
+        self.closing = []
         self.trial_steps = 0           # reservoir steps simulated, all trials
         self.trials_built = 0
         self.fits = 0
@@ -125,7 +129,15 @@ class Evaluator:
         self.wait_seconds = 0.0        # waiting for trial construction
         self.alg_bytes = 0.0           # algorithmic HBM bytes of the simulated steps
 
+    def drain(self):
+        """Wait until every released trial has been destroyed."""
+        for f in self.closing:
+            f.result()
+        self.closing = []
+
     def close(self):
+        self.drain()
+        self.closer.shutdown(wait=True)
         self.pool.shutdown(wait=True)
 
     @staticmethod
@@ -191,7 +203,7 @@ class Evaluator:
                 t.res = t.future.result()
                 t.future = None
             if t.res is not None:
-                t.res.close()
+                self.closing.append(self.closer.submit(t.res.close))
                 t.res = None
 
     def _launch_groups(self, trials):
@@ -253,6 +265,9 @@ class Evaluator:
 
 
 def _stats(ev, t0):
+    drain = getattr(ev, "drain", None)
+    if drain is not None:   # destroys still pending count towards the search's time
+        drain()
     return dict(elapsed=time.perf_counter() - t0, trials_built=ev.trials_built, fits=ev.fits,
                 trial_steps=ev.trial_steps, run_s=round(ev.run_seconds, 3), fit_s=round(ev.fit_seconds, 3),
                 wait_s=round(ev.wait_seconds, 3), alg_bytes=ev.alg_bytes)
