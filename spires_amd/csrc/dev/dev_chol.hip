@@ -1,6 +1,7 @@
-// Batched FP64 ridge systems (the readout fits of a hyperparameter search):
-// many independent normal equations (X_r'X_r + lambda_r I) W_r = X_r'Y_r of one
-// order n, solved together so that every launch spans all systems.
+// FP64 ridge systems in exact mode: one or many independent normal equations
+// (X_r'X_r + lambda_r I) W_r = X_r'Y_r of one order n (a multiple of 64), solved
+// together so that every launch spans all systems (the readout fits of a
+// hyperparameter search, or a single readout fit).
 //
 //  - Gram: lower-triangle tiles of X'X on the FP64 matrix cores
 //    (v_mfma_f64_16x16x4f64, operands straight from the recorded state rows,
@@ -16,8 +17,11 @@
 //    triangle is updated tile by tile.
 //  - The factor kernels also store L' in the (otherwise unused) upper
 //    triangle, so both triangular solves stream coalesced column segments.
-//  - Triangular solves: one workgroup per (system, right-hand side); the
-//    diagonal blocks apply their inverses.
+//  - Triangular solves, up to 8 right-hand sides: one workgroup per (system,
+//    right-hand side); the diagonal blocks apply their inverses.  More
+//    right-hand sides: blocked by 64 rows x 64 columns, one launch applying a
+//    diagonal block's inverse and one subtracting that block's product from
+//    the row blocks still to solve.
 //  - Residual B - A X of the refinement: a batched GEMV over the full
 //    symmetric copy, column chunks summed in a fixed order.
 //
