@@ -265,44 +265,51 @@ __global__ __launch_bounds__(256) void k_chol_panel(int n, int kb, double* A, si
     }
 }
 
-// Trailing update after block kb: A[ib][jb] -= L[ib][kb] L[jb][kb]^T for kb < jb <= ib.
+// Trailing update after block kb: A[ib][jb] -= L[ib][kb] L[jb][kb]^T for kb < jb <= ib, on
+// the FP64 matrix cores, operands straight from the factor (column-major: element
+// (row, col) at col * n + row).  Wave w takes the 32x32 quarter rows r0 = 32(w/2), columns
+// c0 = 32(w%2) of the tile.  MFMA operands A[p][k] = L[j0+c0+p][k+m0+k'] and
+// B[k][q] = L[i0+r0+q][k+m0+k'], so D[p][q] is the product for column c0+p, row r0+q:
+// register t of lane l holds p = 4t + l/16, q = l%16, and the read-modify-write of each
+// register runs along 16 contiguous rows.
 __global__ __launch_bounds__(256) void k_chol_update(int n, int kb, double* A, size_t strideA) {
-    __shared__ double Pi[CB][CB + 1];
-    __shared__ double Pj[CB][CB + 1];
-    const int sys = blockIdx.y, tid = threadIdx.x;
+    const int sys = blockIdx.y;
     int bi, bj;
     tri_decode(blockIdx.x, bi, bj);
     const int k = kb * CB, i0 = (kb + 1 + bi) * CB, j0 = (kb + 1 + bj) * CB;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int r0 = (wave >> 1) * 32, c0 = (wave & 1) * 32;
+    if (i0 == j0 && c0 > r0) return;            // quarter above the diagonal of a diagonal tile
+    const int li = lane & 15, lk = lane >> 4;
     double* As = A + (size_t)sys * strideA;
-    for (int e = tid; e < CB * CB; e += 256) {
-        const int c = e / CB, r = e % CB;
-        Pi[r][c] = As[(size_t)(k + c) * n + i0 + r];
-        Pj[r][c] = As[(size_t)(k + c) * n + j0 + r];
-    }
-    __syncthreads();
-    const int tr = tid & 15, tc = tid >> 4;
-    double acc[4][4] = {};
-    for (int m = 0; m < CB; ++m) {
-        double p[4], v[4];
+    const double* pa = As + (size_t)(k + lk) * n + j0 + c0 + li;   // L[j0+c0+p][k+m], p = li (+16)
+    const double* pb = As + (size_t)(k + lk) * n + i0 + r0 + li;   // L[i0+r0+q][k+m], q = li (+16)
+    d4 acc[2][2];
 #pragma unroll
-        for (int a = 0; a < 4; ++a) p[a] = Pi[tr + 16 * a][m];
+    for (int a = 0; a < 2; ++a)
 #pragma unroll
-        for (int b = 0; b < 4; ++b) v[b] = Pj[tc + 16 * b][m];
-#pragma unroll
-        for (int a = 0; a < 4; ++a)
-#pragma unroll
-            for (int b = 0; b < 4; ++b) acc[a][b] = fma(p[a], v[b], acc[a][b]);
+        for (int b = 0; b < 2; ++b) acc[a][b] = d4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll 4
+    for (int m0 = 0; m0 < CB; m0 += 4) {
+        const size_t off = (size_t)m0 * n;
+        const double a0 = pa[off], a1 = pa[off + 16], b0 = pb[off], b1 = pb[off + 16];
+        acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
+        acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
+        acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
+        acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
     }
 #pragma unroll
-    for (int a = 0; a < 4; ++a)
+    for (int a = 0; a < 2; ++a)
 #pragma unroll
-        for (int b = 0; b < 4; ++b) {
-            const int r = tr + 16 * a, c = tc + 16 * b;
-            if (i0 != j0 || r >= c) {
-                double* p = As + (size_t)(j0 + c) * n + i0 + r;
-                *p = *p - acc[a][b];
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                const int c = c0 + 16 * a + 4 * t + lk, r = r0 + 16 * b + li;
+                if (i0 != j0 || r >= c) {
+                    double* q = As + (size_t)(j0 + c) * n + i0 + r;
+                    *q = *q - acc[a][b][t];
+                }
             }
-        }
 }
 
 constexpr int kSolveThreads = 1024;
