@@ -559,7 +559,8 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs one reservoir per device and has no "
-              "multi-GPU path (see DESIGN.md for how it would shard).", file=sys.stderr)
+              "multi-GPU path (a reservoir would shard by neuron blocks, one spike exchange per step "
+              "over the GPUs).", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np
