@@ -558,7 +558,7 @@ def main():
                     help="run the batched workload (64 uneven reservoirs) instead of the step benchmark")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build runs one reservoir per device and has no "
+        print("bench.py measures one GPU: the library runs one reservoir per device and has no "
               "multi-GPU path (a reservoir would shard by neuron blocks, one spike exchange per step "
               "over the GPUs).", file=sys.stderr)
         sys.exit(2)
