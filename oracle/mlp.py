@@ -2,7 +2,7 @@
 
 TEST INFRASTRUCTURE ONLY: tests/ use it to check the GPU forward, backward and
 Adam kernels (spires_ext_mlp_*); the product never imports it.  It has no
-counterpart in the reference (the surrogate is this build's addition).  All
+counterpart in the reference (the reference has no surrogate model).  All
 arithmetic is float64; the backward pass is pinned against central finite
 differences in tests/test_oracle_mlp.py.
 
