@@ -225,7 +225,7 @@ class Evaluator:
                 out.append((done, cur))
         return out
 
-    def advance(self, trials, F, fit_batch=16):
+    def advance(self, trials, F, fit_batch=32):
         """Run each trial on to fidelity F and score it there.  Returns the
         validation NRMSE of each trial (1e3 for a singular fit or a blow-up)."""
         trials = list(trials)
