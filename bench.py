@@ -3,32 +3,36 @@
 
 Workload (FP64 exact mode, single reservoir): FLIF-GL N=4096, T_mem=512
 (L=512), dt=1, Watts-Strogatz ring k=20 p=0.1 with N(0,1) weights rescaled to
-rho=0.95, input gain 2.0, one input channel u ~ U[0, 0.5].  A "step" is one
-reservoir time step over all N neurons (recurrent CSR pull over the spike
-vector, FLIF-GL update with the 511-lag GL history sum, state row written to
-HBM).  Inputs, weights and the state record are resident in HBM before the
-timed region.  value = N * steps / wall time of the timed steps
-(neuron-steps/s).
+rho=0.95, input gain 2.0, one input channel u ~ U[0, 0.5], T=20000 time steps.
+A reservoir time step is the recurrent CSR pull over the spike vector, the
+FLIF-GL update with the (up to) 511-lag GL history sum, and the state row
+written to HBM.  A bench "step" is a block of T/(steps+warmup) reservoir time
+steps (800 at --steps 20 --warmup 5), run as one persistent launch; the
+warm-up blocks fill the history, so the timed blocks sum all 511 lags.
+Inputs, weights and the state record are resident in HBM before the timed
+region.  value = N * timed time steps / HIP-event time of the timed region
+(neuron-steps/s).  The roofline divides the algorithmic bytes of the lags
+actually summed by the mean launch time; a 1 GiB device-to-device copy, timed
+first, gives the measured copy bandwidth beside the 8 TB/s peak (and, under
+rocprofv3 --pmc, the FETCH_SIZE / WRITE_SIZE calibration).
 
 The CPU baseline is the oracle's C restatement of the reference step
 (oracle/liboracle.so, OpenMP over neurons) on a bounded sample of the same
-workload, timed on this host.
+workload, timed on this host; its first steady-state chunk is compared bit for
+bit with the GPU's records of the same steps.
 
-With --copy-calibration a plain device-to-device copy of 256 MiB runs first:
-its FETCH_SIZE / WRITE_SIZE under rocprofv3 --pmc calibrate the counters that
-give the step kernel's HBM traffic.
-
-With --agile it runs the AGILE hyperparameter search (N=2048, 512 trials) instead: the plain
-batched FP64 search over 512 trials at 18000 steps sets the target NRMSE, and
-successive halving and the surrogate-guided search are timed until they come
-within 2 % of it (value = median GPU-seconds over the surrogate-search seeds).
+The default line also carries the metric's second half, the AGILE search
+(N=2048, 512 trials): the plain batched FP64 search over 512 trials at 18000
+steps sets the target NRMSE, and the surrogate-guided search is timed per seed
+until it comes within 2 % of it (--no-agile leaves it out).  --agile runs the
+full comparison (with successive halving) as its own line.
 
 With --fast it runs FLIF-GL N=16384, L=1024, BA m=8 in FP64 exact and
 FP32 fast mode on the same inputs and reports both rates, raster agreement, the first
 raster divergence, max |dV| before it and each readout's NRMSE (the block size K is
 picked by timing the candidates on a twin reservoir).
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--copy-calibration]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-agile]
                        [--agile [--agile-seeds S]] [--fast [--fast-steps T]] [--batched] [--readout]
                        [--ridge-batch]
 """
@@ -84,31 +88,46 @@ def measured_traffic_per_step():
     return d["hbm_read_bytes_per_step"] + d["hbm_write_bytes_per_step"], d["source"]
 
 
-def cpu_baseline(cfg, csr, W_in, u, budget_s=12.0):
+def cpu_baseline(cfg, csr, W_in, L, budget_s=12.0, check=None):
+    """The oracle's C restatement (OpenMP over neurons) stepping the same
+    workload on this host's cores.  It draws its own input series from the
+    workload's seed (long enough for the history ramp plus the timed sample,
+    whatever the GPU run's length), runs steps 0..L-1 untimed (the history
+    ramp), then times chunks of steady-state steps (full L-1-lag history) until
+    `budget_s` is spent.  `check` = (first GPU state rows, their first step):
+    the oracle's states of the first timed chunk are compared bit for bit with
+    the GPU's records of the same steps."""
     import numpy as np
     from oracle import pyoracle as po
+    from spires_amd import workloads
     po.build()
     n = cfg[0].num_neurons
     params = [cfg[1][i] for i in range(8)]
     o = po.OracleReservoir(po.make_config(n, din=1, dout=cfg[0].num_outputs, gain=cfg[0].input_strength,
                                           neuron="flif_gl", params=params), csr=csr, W_in=W_in)
-    # bounded sample of the same series: steps 0..L-1 (history ramp, untimed), then timed chunks of
-    # steady-state steps (full 511-lag history) until the budget is spent
-    L = 512
+    chunk, cap = 25, 200000
+    u = workloads.flifgl_small_world_inputs(L + cap)
     o.run_record(u[:L], L, states=False, raster=False)
-    chunk, done = 25, 0
-    u = u[L:]
+    done, parity = 0, None
     t0 = time.perf_counter()
-    while True:
-        o.run_record(u[done:done + chunk], chunk, states=True, raster=False)
+    while done + chunk <= cap:
+        X, _, _ = o.run_record(u[L + done:L + done + chunk], chunk, states=True, raster=False)
+        if done == 0 and check is not None:
+            G, row0 = check
+            m = min(chunk, G.shape[0])
+            parity = {"steps": [L, L + m - 1], "bit_exact": bool(row0 == L and np.array_equal(
+                G[:m].view(np.uint64), X[:m].view(np.uint64)))}
         done += chunk
-        if time.perf_counter() - t0 >= budget_s or done + chunk > u.shape[0]:
+        if time.perf_counter() - t0 >= budget_s:
             break
     dt = time.perf_counter() - t0
     cores = int(os.environ.get("OMP_NUM_THREADS", len(os.sched_getaffinity(0))))
-    return {"value": n * done / dt, "unit": "neuron-steps/s", "cores": cores, "kind": "port",
-            "sample": f"steps {L}..{L + done - 1} of the same N={n} L=512 workload (oracle C restatement, "
-                      f"OpenMP over neurons, {dt:.1f} s)"}
+    out = {"value": n * done / dt, "unit": "neuron-steps/s", "cores": cores, "kind": "port",
+           "sample": f"steps {L}..{L + done - 1} of the same N={n} L={L} workload after an untimed "
+                     f"{L}-step history ramp (oracle C restatement, OpenMP over neurons, {dt:.1f} s)"}
+    if parity is not None:
+        out["gpu_parity"] = parity
+    return out
 
 
 def agile_cpu_baseline(budget_s=10.0):
@@ -536,14 +555,190 @@ def run_ridge_batch(args):
     print(json.dumps(line), flush=True)
 
 
+T_CONFIG = 20000          # the single-reservoir workload's run length (time steps)
+CLOCK_GHZ = 2.4           # MI355X peak engine clock
+ADD_LATENCY_CYCLES = 8.5  # dependent v_add_f64 latency (one-wave probe)
+
+
+def copy_bandwidth(hip, nbytes=1 << 30, reps=5):
+    """Device-to-device copy of `nbytes` (hipMemcpyAsync on a stream of its
+    own), best of `reps`: read + write bytes / time, in GB/s."""
+    hip.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+    hip.hipFree.argtypes = [C.c_void_p]
+    hip.hipMemsetAsync.argtypes = [C.c_void_p, C.c_int, C.c_size_t, C.c_void_p]
+    hip.hipMemcpyAsync.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int, C.c_void_p]
+    hip.hipStreamCreate.argtypes = [C.POINTER(C.c_void_p)]
+    hip.hipStreamDestroy.argtypes = [C.c_void_p]
+    hip.hipStreamSynchronize.argtypes = [C.c_void_p]
+    src, dst, st = C.c_void_p(), C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(src), nbytes) == 0 and hip.hipMalloc(C.byref(dst), nbytes) == 0
+    assert hip.hipStreamCreate(C.byref(st)) == 0
+    ev = HipEvents()
+    best = 0.0
+    try:
+        assert hip.hipMemsetAsync(src, 1, nbytes, st) == 0
+        assert hip.hipMemcpyAsync(dst, src, nbytes, 3, st) == 0          # warm-up
+        for _ in range(reps):
+            e0, e1 = ev.create(), ev.create()
+            ev.record(e0, st)
+            assert hip.hipMemcpyAsync(dst, src, nbytes, 3, st) == 0      # hipMemcpyDeviceToDevice
+            ev.record(e1, st)
+            best = max(best, 2.0 * nbytes / (ev.elapsed_ms(e0, e1) * 1e-3) / 1e9)
+    finally:
+        hip.hipStreamSynchronize(st)
+        hip.hipStreamDestroy(st)
+        hip.hipFree(src)
+        hip.hipFree(dst)
+    return best
+
+
+def run_agile_summary(args):
+    """The AGILE half of the metric for the default line: plain search, then the
+    surrogate search over args.agile_seeds seeds (no halving leg)."""
+    import numpy as np
+    from spires_amd import agile_search as ag
+    rungs = (2000, 6000, 18000)
+    th = ag.sample_trials(512, seed=2024)
+    seeds = np.arange(512) + 10000
+    log = (lambda m: print(m, file=sys.stderr, flush=True))
+    ev = ag.Evaluator(n=2048, f_max=rungs[-1], seed=0)
+    plain = ag.plain_search(ev, th, seeds, rungs[-1], log=log)
+    ev.close()
+    target = plain["best"]
+    per_seed = []
+    for sd in range(1, args.agile_seeds + 1):
+        ev = ag.Evaluator(n=2048, f_max=rungs[-1], seed=0)
+        r = ag.surrogate_search(ev, rungs, target=target, seed=sd, log=log)
+        ev.close()
+        matched = r["best"] <= 1.02 * target
+        per_seed.append({"seed": sd, "gpu_s": r["elapsed"], "best": r["best"], "matched": bool(matched),
+                         "trials": r.get("trials")})
+    times = sorted(x["gpu_s"] if x["matched"] else math.inf for x in per_seed)
+    med = float(np.median(times))
+    return {"unit": "GPU-s to matched val NRMSE (median over surrogate-search seeds; unmatched = never)",
+            "value": med, "plain_gpu_s": plain["elapsed"], "target_nrmse": target,
+            "speedup_median": plain["elapsed"] / med if med > 0 else None,
+            "matched": f"{sum(x['matched'] for x in per_seed)}/{len(per_seed)}",
+            "workload": "FLIF-GL N=2048, 512 trials, rungs 2000/6000/18000, 84-step-ahead Mackey-Glass, "
+                        "6 hyperparameters, surrogate 7-64-64-1 FP32 Adam lr=1e-3; plain = 512 x 18000 steps",
+            "seeds": per_seed}
+
+
+def run_step(args):
+    """The single-reservoir workload (FP64 exact FLIF-GL N=4096 L=512, T=20000).
+    A bench step is a block of T/(steps+warmup) reservoir time steps (one
+    persistent launch); the warm-up blocks fill the 511-lag history, so the timed
+    blocks run in steady state.  When the warm-up blocks are too short for that,
+    an untimed ramp runs first."""
+    import numpy as np
+    import spires_amd as sa
+    from spires_amd import workloads
+    K, W = args.steps, args.warmup
+    L = 512
+    block = max(1, T_CONFIG // max(1, K + W))
+    ramp = max(0, (L - 1) - W * block)
+    T = ramp + (K + W) * block
+    cfg, csr, W_in, u = workloads.flifgl_small_world(T=T)
+    n = cfg[0].num_neurons
+    ev = HipEvents()
+    copy_gbs = copy_bandwidth(ev.hip)
+    r = sa.Reservoir(cfg, csr=csr, W_in=W_in)
+    plan = r.kernel_plan()
+    r.upload_inputs(u)
+    flags = sa.REC_STATE | sa.REC_APPEND
+    r.reserve_records(T, flags)
+    stream = r.stream()
+    t = 0
+    if ramp:
+        r.run_resident(0, ramp, flags)
+        t = ramp
+    for _ in range(W):
+        r.run_resident(t, block, flags)
+        t += block
+    r.sync()
+    t_timed0 = t
+    marks = [(ev.create(), ev.create()) for _ in range(K)]
+    w0 = time.perf_counter()
+    for k in range(K):
+        ev.record(marks[k][0], stream)
+        r.run_resident(t, block, flags)
+        ev.record(marks[k][1], stream)
+        t += block
+    r.sync()
+    wall = time.perf_counter() - w0
+    region_ms = ev.elapsed_ms(marks[0][0], marks[-1][1])
+    launch_ms = np.array([ev.elapsed_ms(a, b) for a, b in marks])
+    steps_timed = K * block
+    value = n * steps_timed / (region_ms * 1e-3)
+    nnz = len(csr[2])
+    # algorithmic bytes of the lags actually summed: step st sums min(st, L-1) lags
+    per_lag = 8 * n
+    fixed = (workloads.algorithmic_bytes_per_neuron_step(n, nnz, L) - 8 * (L - 1)) * n
+    lags = np.minimum(np.arange(t_timed0, t_timed0 + steps_timed), L - 1)
+    alg_bytes_timed = float(per_lag * lags.sum() + fixed * steps_timed)
+    alg_per_launch = alg_bytes_timed / K
+    achieved = alg_per_launch / (float(launch_ms.mean()) * 1e-3) / 1e9
+    us_step_med = float(np.median(launch_ms)) * 1e3 / block
+    floor_us = (L - 1) * ADD_LATENCY_CYCLES / (CLOCK_GHZ * 1e3)
+    traffic_step, traffic_src = measured_traffic_per_step()
+    G, _, _ = r.read_records(L, 25, states=True) if T >= L + 25 else (None, None, None)
+    X, _, _ = r.read_records(T - 1, 1, states=True)
+    line = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "neuron-steps/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": W,
+        "ms_per_step": region_ms / K,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "FLIF-GL reservoir, FP64 exact mode: N=4096, T_mem=512 (L=512), dt=1, "
+                               "Watts-Strogatz k=20 p=0.1, N(0,1) weights rho=0.95, gain 2.0, u~U[0,0.5], T=%d" % T,
+                   "neurons": n, "memory_len": L, "nnz": nnz, "time_steps": T, "mode": "exact",
+                   "bench_step": "%d reservoir time steps (one persistent launch)" % block,
+                   "untimed_ramp_steps": ramp, "timed_time_steps": [t_timed0, t_timed0 + steps_timed - 1],
+                   "record": "state V [T x N] f64"},
+        "us_per_time_step": {"region": region_ms * 1e3 / steps_timed, "launch_median": us_step_med,
+                             "launch_min": float(launch_ms.min()) * 1e3 / block,
+                             "launch_max": float(launch_ms.max()) * 1e3 / block, "launches": K},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "frac_of_measured_copy": achieved / copy_gbs if copy_gbs else None,
+                     "measured_copy_GBs": copy_gbs,
+                     "traffic": traffic_step * block if traffic_step else None,
+                     "traffic_source": traffic_src,
+                     "algorithmic_bytes_per_launch": alg_per_launch,
+                     "kernel": "k_persist_tp<%d> (one launch per bench step, %d workgroups)" % (plan["nb"],
+                                                                                            plan["workgroups"])},
+        "latency_floor": {"bound": "dependent FP64 add chain of the GL sum",
+                          "floor_us_per_time_step": floor_us, "frac": floor_us / us_step_med,
+                          "basis": "%d lags x %.1f cycles at %.1f GHz" % (L - 1, ADD_LATENCY_CYCLES, CLOCK_GHZ)},
+        "wall_s": wall,
+        "final_state_checksum": float(np.sum(X)),
+    }
+    r.close()
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline(cfg, csr, W_in, L, check=(G, L) if G is not None else None)
+    if not args.no_agile:
+        try:
+            line["agile"] = run_agile_summary(args)
+        except Exception as e:           # the step line is still reported
+            line["agile"] = {"error": repr(e)}
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20000)
-    ap.add_argument("--warmup", type=int, default=200)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--copy-calibration", action="store_true",
-                    help="first copy 256 MiB device-to-device (counter calibration for rocprofv3 --pmc)")
+    ap.add_argument("--no-agile", action="store_true",
+                    help="leave the AGILE search (the metric's second half) out of the default line")
     ap.add_argument("--agile", action="store_true",
                     help="run the AGILE hyperparameter search comparison instead of the step benchmark")
     ap.add_argument("--agile-seeds", type=int, default=12)
@@ -584,71 +779,7 @@ def main():
     if args.ridge_batch:
         run_ridge_batch(args)
         return
-    if args.copy_calibration:
-        import torch
-        nbytes = 256 << 20
-        src = torch.ones(nbytes // 8, dtype=torch.float64, device="cuda")
-        dst = torch.empty_like(src)
-        dst.copy_(src)
-        torch.cuda.synchronize()
-        print(json.dumps({"copy_calibration_bytes": nbytes}), flush=True)
-        del src, dst
-    K, W = args.steps, args.warmup
-    cfg, csr, W_in, u = workloads.flifgl_small_world(T=K + W)
-    n = cfg[0].num_neurons
-    L = 512
-    r = sa.Reservoir(cfg, csr=csr, W_in=W_in)
-    plan = r.kernel_plan()
-    r.upload_inputs(u)
-    r.reserve_records(max(K, W), sa.REC_STATE)
-    ev = HipEvents()
-    stream = r.stream()
-    r.run_resident(0, W, sa.REC_STATE)
-    r.sync()
-    e0, e1 = ev.create(), ev.create()
-    ev.record(e0, stream)
-    t0 = time.perf_counter()
-    r.run_resident(W, K, sa.REC_STATE)
-    ev.record(e1, stream)
-    r.sync()
-    wall = time.perf_counter() - t0
-    gpu_ms = ev.elapsed_ms(e0, e1)
-    ms_per_step = gpu_ms / K
-    value = n * K / (gpu_ms * 1e-3)
-    X, _, _ = r.read_records(K - 1, 1, states=True)
-    nnz = len(csr[2])
-    bps = workloads.algorithmic_bytes_per_neuron_step(n, nnz, L)
-    achieved = bps * n / (ms_per_step * 1e-3) / 1e9
-    traffic_step, traffic_src = measured_traffic_per_step()
-    line = {
-        "metric": METRIC,
-        "value": value,
-        "unit": "neuron-steps/s",
-        "n_gpus": 1,
-        "steps": K,
-        "warmup": W,
-        "ms_per_step": ms_per_step,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f64",
-        "data": "synthetic",
-        "config": {"workload": "FLIF-GL reservoir, FP64 exact mode: N=4096, T_mem=512 (L=512), dt=1, "
-                               "Watts-Strogatz k=20 p=0.1, N(0,1) weights rho=0.95, gain 2.0, u~U[0,0.5]",
-                   "neurons": n, "memory_len": L, "nnz": nnz, "time_steps": K, "mode": "exact",
-                   "record": "state V [T x N] f64"},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": traffic_step * K if traffic_step else None,
-                     "traffic_source": traffic_src,
-                     "algorithmic_bytes_per_launch": bps * n * K,
-                     "kernel": "k_persist_tp<%d> (one launch of K steps, %d workgroups)" % (plan["nb"], plan["workgroups"])},
-        "wall_s": wall,
-        "final_state_checksum": float(np.sum(X)),
-    }
-    if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(cfg, csr, W_in, u)
-    print(json.dumps(line), flush=True)
+    run_step(args)
 
 
 if __name__ == "__main__":

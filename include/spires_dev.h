@@ -26,6 +26,7 @@ extern "C" {
 #define SPIRES_DEV_EALLOC    1
 #define SPIRES_DEV_EINTERNAL 2
 #define SPIRES_DEV_EINVAL    3
+#define SPIRES_DEV_ESINGULAR 4   /* ridge system singular: W_out left unchanged */
 
 typedef struct spires_dev spires_dev;
 
@@ -162,6 +163,10 @@ int  spires_dev_sync(spires_dev *g);
 void *spires_dev_stream(spires_dev *g);
 size_t spires_dev_steps(const spires_dev *g);
 const char *spires_dev_last_error(void);
+/* Release what the layer holds beyond any reservoir: the per-device rocBLAS
+ * handle and readout workspace.  Call with no reservoir work in flight (the
+ * next readout call re-creates them). */
+void spires_dev_shutdown(void);
 
 /* coarse_grain_reservoir (reservoir.c:782-967) on the current device: the
  * reference's greedy merge passes over the CSR matrix (rp, ci, val) with

@@ -162,6 +162,11 @@ spires_status spires_ext_get_csr(const spires_reservoir *r, int64_t *row_ptr,
                                  int32_t *col_idx, double *values);
 spires_status spires_ext_get_input_weights(const spires_reservoir *r, double *W_in);
 const char   *spires_ext_last_error(void);
+/* Releases the engine's process-wide GPU resources (the per-device rocBLAS
+ * handle and readout workspace); call after every reservoir is destroyed,
+ * before the process exits.  Later calls re-create them on demand.  No
+ * counterpart in the reference, whose library holds no GPU state. */
+void          spires_ext_shutdown(void);
 
 /* AGILE (Levy exploration + SPSA refinement, the search behind
  * spires_optimize) on a caller-supplied loss.  opts[11] = {mu, eta_min,

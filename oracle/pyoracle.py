@@ -216,6 +216,8 @@ class OracleReservoir:
 
     def run_record(self, u, T, states=True, raster=True, outputs=False):
         u = None if u is None else np.ascontiguousarray(u, np.float64)
+        if u is not None and u.size < T * self.Din:
+            raise ValueError(f"run_record: {T} steps need {T * self.Din} input values, got {u.size}")
         X = np.empty((T, self.N)) if states else None
         R = np.empty((T, self.N), np.uint8) if raster else None
         Y = np.empty((T, self.Dout)) if outputs else None

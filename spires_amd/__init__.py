@@ -7,9 +7,11 @@ HIP library is missing, ``lib()`` raises.
 """
 from __future__ import annotations
 
+import atexit
 import ctypes as C
 import os
 import subprocess
+import weakref
 
 import numpy as np
 
@@ -123,6 +125,7 @@ SIGNATURES = {
     "spires_ext_get_csr": (C.c_int, [_P, _I64, _I32, _D]),
     "spires_ext_get_input_weights": (C.c_int, [_P, _D]),
     "spires_ext_last_error": (C.c_char_p, []),
+    "spires_ext_shutdown": (None, []),
     "spires_ext_agile_optimize": (C.c_int, [_D, _D, _D, C.c_int, LOSS_FN, _P, _D, _D]),
     "spires_ext_spectral_radius": (C.c_double, [_SZ, _I64, _I32, _D]),
     "spires_ext_mlp_create": (C.c_int, [C.c_int, C.c_int, C.c_ulonglong, C.POINTER(_P)]),
@@ -146,6 +149,26 @@ SIGNATURES = {
 _lib = None
 _libc = C.CDLL("libc.so.6")
 _libc.free.argtypes = [_P]
+
+# Live handle owners (reservoirs, time blocks, surrogates), closed at exit
+# before the engine releases its process-wide GPU resources: nothing of ours
+# is left for the HIP runtime's own teardown.
+_live = weakref.WeakSet()
+
+
+def shutdown() -> None:
+    """Close every live handle, then release the engine's per-device rocBLAS
+    handle and readout workspace (spires_ext_shutdown).  Runs at exit."""
+    for obj in list(_live):
+        try:
+            obj.close()
+        except Exception:
+            pass
+    if _lib is not None:
+        _lib.spires_ext_shutdown()
+
+
+atexit.register(shutdown)
 
 
 def build(force: bool = True) -> None:
@@ -313,6 +336,7 @@ class TimeBlock:
         self.h = _P()
         _check(lib().spires_ext_timeblock_create(arr, len(self._res), K, flags, C.byref(self.h)))
         self.K = K
+        _live.add(self)
 
     @property
     def launches(self):
@@ -342,6 +366,7 @@ class Surrogate:
             raise RuntimeError("spires_ext_mlp_create: " + lib().spires_ext_mlp_last_error().decode())
         self.in_dim = in_dim
         self.count = lib().spires_ext_mlp_num_params(self.h)
+        _live.add(self)
 
     def _f32(self, a):
         return self._np.ascontiguousarray(a, dtype=self._np.float32)
@@ -423,11 +448,13 @@ class Reservoir:
                                            self._ci.ctypes.data_as(_I32), _dp(self._val), _dp(wi), _dp(wo),
                                            C.byref(h)))
         self.h = h
+        _live.add(self)
 
     @classmethod
     def _wrap(cls, h, keep, N, Din, Dout):
         r = cls.__new__(cls)
         r.h, r._keep, r.N, r.Din, r.Dout = h, keep, N, Din, Dout
+        _live.add(r)
         return r
 
     def close(self):

@@ -36,12 +36,34 @@ struct ReadoutCtx {
     rocblas_int* iw = nullptr; size_t iw_cap = 0;
 };
 
+constexpr int kMaxDevices = 64;
+static ReadoutCtx g_readout[kMaxDevices];
+
 static ReadoutCtx* readout_ctx() {
-    constexpr int kMaxDevices = 64;
-    static ReadoutCtx ctx[kMaxDevices];
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) return nullptr;
-    return &ctx[dev];
+    return &g_readout[dev];
+}
+
+// Destroys every device's handle and frees its workspace, each with its device
+// current; the process-exit path then finds no rocBLAS or HIP object of ours alive.
+void readout_shutdown() {
+    int cur = 0;
+    const bool have_cur = hipGetDevice(&cur) == hipSuccess;
+    for (int d = 0; d < kMaxDevices; ++d) {
+        ReadoutCtx& c = g_readout[d];
+        std::lock_guard<std::mutex> lk(c.mu);
+        if (!c.h && !c.ws && !c.iw) continue;
+        if (hipSetDevice(d) != hipSuccess) continue;
+        (void)hipDeviceSynchronize();
+        if (c.h) rocblas_destroy_handle(c.h);
+        c.h = nullptr;
+        dfree(c.ws);
+        dfree(c.iw);
+        c.ws = nullptr; c.ws_cap = 0;
+        c.iw = nullptr; c.iw_cap = 0;
+    }
+    if (have_cur) (void)hipSetDevice(cur);
 }
 
 // Locks the current device's context, creates its handle on first use and
@@ -259,6 +281,8 @@ using namespace sdev;
 
 extern "C" {
 
+void spires_dev_shutdown(void) { readout_shutdown(); }
+
 static int ridge_batch_impl(spires_dev* const* gs, int count, size_t row0, size_t row1, const double* const* Y,
                             const double* lambda, double* info_out);
 
@@ -287,7 +311,7 @@ int spires_dev_ridge_rows(spires_dev* g, size_t row0, size_t row1, const double*
         }
         if (inf[0] < 0) {   // W_out left unchanged, as the reference does
             set_error("ridge system is singular");
-            return SPIRES_DEV_EINTERNAL;
+            return SPIRES_DEV_ESINGULAR;
         }
         return 0;
     }
@@ -394,7 +418,7 @@ int spires_dev_ridge_rows(spires_dev* g, size_t row0, size_t row1, const double*
             cleanup();
             set_error("ridge system is singular");
             if (info_out) info_out[0] = (double)i2;
-            return SPIRES_DEV_EINTERNAL;   // W_out left unchanged, as the reference does
+            return SPIRES_DEV_ESINGULAR;   // W_out left unchanged, as the reference does
         }
         rocsolver_dgetrs(h, rocblas_operation_none, n, D, A, n, ipiv, X, n);
     } else {

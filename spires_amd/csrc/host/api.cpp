@@ -432,31 +432,46 @@ spires_status spires_train_online(spires_reservoir* r, const double* target_vec,
     return dev_status(spires_dev_online(r->dev, target_vec, lr));
 }
 
-spires_status spires_ext_train_ridge_washout(spires_reservoir* r, const double* u, const double* y, size_t T,
-                                             size_t washout, double lambda, double* info) {
-    if (!r || !r->dev || !u || !y) return SPIRES_ERR_INVALID_ARG;
-    spires_status s = spires_reservoir_reset(r);
-    if (s != SPIRES_OK) return s;
+namespace {
+// Reset, run u, drop `washout` states, fit W_out.  Returns the device layer's
+// code; a failed reset returns -1 with its status in *reset_status.
+int ridge_washout_dev(spires_reservoir* r, const double* u, const double* y, size_t T, size_t washout,
+                      double lambda, double* info, spires_status* reset_status) {
+    *reset_status = spires_reservoir_reset(r);
+    if (*reset_status != SPIRES_OK) return -1;
     if (T <= washout) {
         // empty design matrix: A = lambda I, b = 0 -> W_out = 0 (reference LU of lambda*I)
         if (lambda != 0.0) {
             std::vector<double> z(r->cfg.num_neurons * r->cfg.num_outputs, 0.0);
-            if (!z.empty()) return dev_status(spires_dev_set_readout(r->dev, z.data()));
+            if (!z.empty()) return spires_dev_set_readout(r->dev, z.data());
         }
-        return SPIRES_OK;
+        return 0;
     }
     int rc = spires_dev_upload_inputs(r->dev, u, T);
     if (!rc) rc = spires_dev_run(r->dev, 0, T, SPIRES_DEV_REC_STATE);
     if (!rc) rc = spires_dev_ridge(r->dev, washout, y, lambda, info);
-    return dev_status(rc);
+    return rc;
+}
+}  // namespace
+
+spires_status spires_ext_train_ridge_washout(spires_reservoir* r, const double* u, const double* y, size_t T,
+                                             size_t washout, double lambda, double* info) {
+    if (!r || !r->dev || !u || !y) return SPIRES_ERR_INVALID_ARG;
+    spires_status s = SPIRES_OK;
+    const int rc = ridge_washout_dev(r, u, y, T, washout, lambda, info, &s);
+    return rc < 0 ? s : dev_status(rc);
 }
 
 spires_status spires_train_ridge(spires_reservoir* r, const double* input_series, const double* target_series,
                                  size_t series_length, double lambda) {
     if (!r || !r->dev || !input_series || !target_series) return SPIRES_ERR_INVALID_ARG;
-    spires_status s = spires_ext_train_ridge_washout(r, input_series, target_series, series_length, 0, lambda, nullptr);
-    // the reference reports success even when an output's solve fails (reservoir.c:663-667)
-    return s == SPIRES_ERR_INTERNAL ? SPIRES_OK : s;
+    spires_status s = SPIRES_OK;
+    const int rc = ridge_washout_dev(r, input_series, target_series, series_length, 0, lambda, nullptr, &s);
+    if (rc < 0) return s;
+    // The reference reports success when an output's solve fails (reservoir.c:663-667) and
+    // keeps that W_out; run, HIP and exchange-timeout failures still reach the caller.
+    if (rc == SPIRES_DEV_ESINGULAR) { set_error(spires_dev_last_error()); return SPIRES_OK; }
+    return dev_status(rc);
 }
 
 spires_status spires_train_rls(spires_reservoir* r, const double* input_series, const double* target_series,
@@ -731,6 +746,8 @@ spires_status spires_ext_get_input_weights(const spires_reservoir* r, double* W_
 }
 
 const char* spires_ext_last_error(void) { return last_error().c_str(); }
+
+void spires_ext_shutdown(void) { spires_dev_shutdown(); }
 
 double spires_ext_spectral_radius(size_t n, const int64_t* row_ptr, const int32_t* col_idx, const double* values) {
     if (!row_ptr || n == 0) return 0.0;

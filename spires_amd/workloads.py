@@ -48,8 +48,14 @@ def flifgl_small_world(n=4096, T=20000, T_mem=512, seed=1):
     params = flif_gl_params(alpha=0.8, T_mem=T_mem, bias=0.05, t_ref=2.0)
     cfg, csr, W_in = gaussian_reservoir(n, "small_world", 20.0 / (n - 1), 0.95, params, seed, din=1,
                                         dout=100, gain=2.0)
-    u = np.random.default_rng(seed + 1000).uniform(0.0, 0.5, (T, 1))
-    return cfg, csr, W_in, u
+    return cfg, csr, W_in, flifgl_small_world_inputs(T, seed)
+
+
+def flifgl_small_world_inputs(T, seed=1):
+    """The input series of flifgl_small_world: u ~ U[0, 0.5], [T x 1].  Drawn
+    sequentially from one generator, so a shorter series is a prefix of a
+    longer one."""
+    return np.random.default_rng(seed + 1000).uniform(0.0, 0.5, (T, 1))
 
 
 def algorithmic_bytes_per_neuron_step(n, nnz, L):
