@@ -191,6 +191,13 @@ int  spires_dev_random_er(int n, double p, double ei, unsigned long long seed, d
  * CSR matrix; *lambda is bit-identical to the CPU loop's, *iters its count. */
 int  spires_dev_spectral_radius(int n, const int64_t *row_ptr, const int32_t *col_idx,
                                 const double *values, double *lambda, int *iters);
+/* spires_dev_spectral_radius for a CSR in device memory, allocating nothing:
+ * the work buffers are a fixed per-device array (n <= 19968), *lambda and
+ * *iters are device pointers written in stream order on `stream` (a
+ * hipStream_t, NULL = null stream), and the call does not wait.  Calls that
+ * overlap on one device race on the shared array. */
+int  spires_dev_spectral_radius_noalloc(int n, const int64_t *row_ptr, const int32_t *col_idx,
+                                        const double *values, double *lambda, int *iters, void *stream);
 
 #ifdef __cplusplus
 }

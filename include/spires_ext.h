@@ -70,6 +70,21 @@ spires_status spires_ext_create_device(const spires_reservoir_config *cfg, uint6
 spires_status spires_ext_spectral_radius_device(size_t n, const int64_t *row_ptr,
                                                 const int32_t *col_idx, const double *values,
                                                 double *lambda, int *iters);
+/* spires_ext_spectral_radius_device without any allocation.  No counterpart
+ * in the reference: it is for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  d_row_ptr [n+1], d_col_idx and d_values are a valid CSR
+ * (monotone row pointers, columns in [0, n)) in device memory of the current
+ * device; *d_lambda (double) and *d_iters (int) are device memory too, written
+ * in stream order on `stream` (a hipStream_t; NULL = the null stream); the
+ * call returns without waiting.  Same operations in the same order as
+ * spires_ext_spectral_radius_device, so lambda is bit-identical.  The work
+ * buffers are one fixed array of 328,856 bytes per device, held until the
+ * process ends: n above 19968 is rejected with SPIRES_ERR_INVALID_ARG, and
+ * calls that overlap on one device race on that array. */
+spires_status spires_ext_spectral_radius_device_noalloc(size_t n, const int64_t *d_row_ptr,
+                                                        const int32_t *d_col_idx, const double *d_values,
+                                                        double *d_lambda, int *d_iters, void *stream);
 
 /* Run T macro steps over u [T x Din] and copy back any of: states [T x N]
  * (V after each step), raster [T x N] (u8 spikes), outputs [T x Dout]. */

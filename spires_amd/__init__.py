@@ -142,6 +142,7 @@ SIGNATURES = {
                                        C.POINTER(_I32), C.POINTER(_D), C.POINTER(C.c_double)]),
     "spires_ext_spectral_radius_device": (C.c_int, [_SZ, _I64, _I32, _D, C.POINTER(C.c_double),
                                                     C.POINTER(C.c_int)]),
+    "spires_ext_spectral_radius_device_noalloc": (C.c_int, [_SZ, _P, _P, _P, _P, _P, _P]),
     "spires_ext_build_weights": (C.c_int, [C.POINTER(Config), C.POINTER(_SZ), C.POINTER(_I64), C.POINTER(_I32),
                                            C.POINTER(_D), C.POINTER(_D), C.POINTER(_D)]),
 }

@@ -11,9 +11,10 @@
 //
 // Spectral radius: the reference's power iteration (sparse.c:93-127) -- x = 1,
 // y = W x with each row summed in CSR order, lambda = max |y|, x = y / lambda,
-// stop when |lambda - lambda_prev| < 1e-6 or after 1000 iterations -- in one
-// workgroup (rows are owned by threads, the max is a workgroup reduction), so
-// every operation and its order match the CPU loop and lambda is bit-identical.
+// stop when |lambda - lambda_prev| < 1e-6 or after 1000 iterations -- one
+// launch per iteration (rows are owned by threads, the max is a reduction over
+// workgroup maxima), so every operation and its order match the CPU loop and
+// lambda is bit-identical.
 // The rescale W *= rho / lambda (when lambda > 1e-9) follows reservoir.c:548-557
 // and runs on the device too, so only the finished CSR crosses PCIe.
 #include "dev_internal.hpp"
@@ -22,6 +23,8 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
+#include <string>
 #include <vector>
 
 namespace sdev {
@@ -148,6 +151,33 @@ __global__ void __launch_bounds__(kPowerRows) k_power_step(int it, int n, const 
     if (tid == 0) w.part[(size_t)(it & 1) * groups + blockIdx.x] = red[0];
 }
 
+// Doubles of power-iteration work buffers for n rows: y, part, lam_hist, lam, state.
+constexpr size_t power_ws_doubles(int n) {
+    return 2 * (size_t)n + 2 * (size_t)((n + kPowerRows - 1) / kPowerRows) + 1001 + 1 + 1;
+}
+
+// Carves the work buffers out of ws (power_ws_doubles(n) doubles).
+void power_ws_layout(double* ws, int n, PowerWs* w) {
+    const int groups = (n + kPowerRows - 1) / kPowerRows;
+    w->y = ws;
+    w->part = w->y + 2 * (size_t)n;
+    w->lam_hist = w->part + 2 * (size_t)groups;
+    w->lam = w->lam_hist + 1001;
+    w->state = (int*)(w->lam + 1);
+}
+
+// The 1001 iteration launches (w.state must be zero when the first one runs).
+int power_iter_launch(hipStream_t st, int n, const int64_t* rp, const int32_t* col, const double* val,
+                      const PowerWs& w) {
+    const size_t lds = (size_t)n * sizeof(double);
+    const int groups = (n + kPowerRows - 1) / kPowerRows;
+    SDEV_CK(hipFuncSetAttribute((const void*)k_power_step, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    for (int it = 0; it <= 1000; ++it)
+        hipLaunchKernelGGL(k_power_step, dim3(groups), dim3(kPowerRows), lds, st, it, n, rp, col, val, w);
+    SDEV_CK(hipGetLastError());
+    return 0;
+}
+
 // Enqueue the power iteration of the device CSR (rp, col, val) on st; *w.lam
 // and w.state[1] hold lambda and the iteration count once st drains.  The
 // workspace comes from one allocation released by the caller after the sync.
@@ -157,24 +187,60 @@ int power_iter_enqueue(hipStream_t st, int n, const int64_t* rp, const int32_t* 
         set_error("power iteration: n above 19968 does not fit x in LDS");
         return SPIRES_DEV_EINVAL;
     }
-    const size_t lds = (size_t)n * sizeof(double);
-    const int groups = (n + kPowerRows - 1) / kPowerRows;
-    const size_t doubles = 2 * (size_t)n + 2 * (size_t)groups + 1001 + 1 + 1;   // y, part, lam_hist, lam, state
     double* ws = nullptr;
-    int rc = dalloc(&ws, doubles);
+    int rc = dalloc(&ws, power_ws_doubles(n));
     if (rc) return rc;
     *ws_out = ws;
-    w->y = ws;
-    w->part = w->y + 2 * (size_t)n;
-    w->lam_hist = w->part + 2 * (size_t)groups;
-    w->lam = w->lam_hist + 1001;
-    w->state = (int*)(w->lam + 1);
+    power_ws_layout(ws, n, w);
     SDEV_CK(hipMemsetAsync(w->state, 0, 2 * sizeof(int), st));
-    SDEV_CK(hipFuncSetAttribute((const void*)k_power_step, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    for (int it = 0; it <= 1000; ++it)
-        hipLaunchKernelGGL(k_power_step, dim3(groups), dim3(kPowerRows), lds, st, it, n, rp, col, val, *w);
-    SDEV_CK(hipGetLastError());
+    return power_iter_launch(st, n, rp, col, val, *w);
+}
+
+// ---- Power iteration without allocation ------------------------------------
+// The work buffers of spires_dev_spectral_radius_noalloc live in one fixed
+// array per device, sized for the largest n the kernel takes (kPowerMaxN:
+// 2 x 19968 + 2 x 78 + 1003 doubles = 328,856 bytes); each device holds its
+// copy until the process ends.  The host never passes `scratch` itself to a
+// kernel: its device address comes from hipGetSymbolAddress, once per device.
+constexpr size_t kScratchDoubles = power_ws_doubles(kPowerMaxN);
+static __device__ __align__(256) double scratch[kScratchDoubles];
+
+constexpr int kMaxScratchDevices = 64;
+static std::mutex g_scratch_mu;
+static double* g_scratch_ptr[kMaxScratchDevices] = {};
+
+// Device address of the current device's scratch array (looked up on the
+// first call per device, kept afterwards).
+static int scratch_address(double** out) {
+    int dev = 0;
+    SDEV_CK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxScratchDevices) {
+        set_error("power iteration scratch: device index out of range");
+        return SPIRES_DEV_EINVAL;
+    }
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            (void)hipGetLastError();
+            set_error(std::string("hipGetSymbolAddress(scratch) failed: ") + hipGetErrorString(e));
+            return SPIRES_DEV_EINTERNAL;
+        }
+        g_scratch_ptr[dev] = (double*)p;
+    }
+    *out = g_scratch_ptr[dev];
     return 0;
+}
+
+// Clears the two state words (a kernel, not a memset, so the call can be captured in a graph).
+__global__ void k_power_clear(int* __restrict__ state) {
+    if (threadIdx.x < 2) state[threadIdx.x] = 0;
+}
+
+// Copies the iteration count out of the state words.
+__global__ void k_power_iters_out(const int* __restrict__ state, int* __restrict__ iters) {
+    if (threadIdx.x == 0) *iters = state[1];
 }
 
 __global__ void k_rescale(size_t nnz, double rho, const double* __restrict__ lam, double* __restrict__ val) {
@@ -290,6 +356,28 @@ int spires_dev_spectral_radius(int n, const int64_t* rp_h, const int32_t* col_h,
     if (e != hipSuccess) return done(hip_fail(e, "power iteration", __LINE__));
     if (iters) *iters = hs[1];
     return done(0);
+}
+
+int spires_dev_spectral_radius_noalloc(int n, const int64_t* rp, const int32_t* col, const double* val,
+                                       double* lam, int* iters, void* stream) {
+    if (n < 1 || !rp || !col || !val || !lam || !iters) return SPIRES_DEV_EINVAL;
+    if (n > kPowerMaxN) {
+        set_error("spectral_radius_noalloc: n above 19968 exceeds the fixed work buffers");
+        return SPIRES_DEV_EINVAL;
+    }
+    double* ws = nullptr;
+    int rc = scratch_address(&ws);
+    if (rc) return rc;
+    PowerWs w;
+    power_ws_layout(ws, n, &w);
+    w.lam = lam;                         // the result goes straight to the caller's buffer
+    hipStream_t st = (hipStream_t)stream;
+    hipLaunchKernelGGL(k_power_clear, dim3(1), dim3(64), 0, st, w.state);
+    SDEV_CK(hipGetLastError());
+    if ((rc = power_iter_launch(st, n, rp, col, val, w))) return rc;
+    hipLaunchKernelGGL(k_power_iters_out, dim3(1), dim3(64), 0, st, (const int*)w.state, iters);
+    SDEV_CK(hipGetLastError());
+    return 0;
 }
 
 }  // extern "C"

@@ -814,6 +814,15 @@ spires_status spires_ext_spectral_radius_device(size_t n, const int64_t* row_ptr
     return dev_status(spires_dev_spectral_radius((int)n, row_ptr, col_idx, values, lambda, iters));
 }
 
+spires_status spires_ext_spectral_radius_device_noalloc(size_t n, const int64_t* d_row_ptr,
+                                                        const int32_t* d_col_idx, const double* d_values,
+                                                        double* d_lambda, int* d_iters, void* stream) {
+    if (n == 0 || !d_row_ptr || !d_col_idx || !d_values || !d_lambda || !d_iters) return SPIRES_ERR_INVALID_ARG;
+    if (n > (size_t)INT32_MAX) { set_error("spectral_radius_noalloc: n too large"); return SPIRES_ERR_INVALID_ARG; }
+    return dev_status(spires_dev_spectral_radius_noalloc((int)n, d_row_ptr, d_col_idx, d_values, d_lambda, d_iters,
+                                                         stream));
+}
+
 spires_status spires_ext_build_weights(const spires_reservoir_config* cfg, size_t* nnz_out, int64_t** row_ptr,
                                        int32_t** col_idx, double** values, double** W_in, double** W_out) {
     if (!cfg || !nnz_out || !row_ptr || !col_idx || !values || !W_in || !W_out) return SPIRES_ERR_INVALID_ARG;
