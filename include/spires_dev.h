@@ -117,6 +117,10 @@ void spires_dev_timeblock_destroy(spires_dev_timeblock *tb);
  * 1 column bitmaps), [4] recurrent weights held in LDS, [5] LDS bytes per
  * workgroup; [6] neurons per workgroup in a batched launch (0: cannot batch). */
 int  spires_dev_plan_info(spires_dev *g, int *info);
+/* Raise the exchange abort flag, as a timed-out spike-exchange wait would:
+ * the next persistent launch gives up at its first unfinished wait and the
+ * launch reports the timeout (fault injection for error-path tests). */
+int  spires_dev_inject_exchange_timeout(spires_dev *g);
 /* Pre-size the record buffers (so a later run allocates nothing). */
 int  spires_dev_reserve_records(spires_dev *g, size_t rows, unsigned flags);
 int  spires_dev_read_records(spires_dev *g, size_t row0, size_t rows, double *states,

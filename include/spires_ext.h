@@ -157,6 +157,13 @@ void          spires_ext_timeblock_destroy(spires_timeblock *tb);
  * in LDS, LDS bytes, neurons per workgroup when batched).  No counterpart in
  * the reference. */
 spires_status spires_ext_kernel_plan(spires_reservoir *r, int *info);
+/* Fault injection for callers' error-path tests: the next persistent run of
+ * r behaves as if a spike-exchange wait had timed out (the kernel gives up at
+ * its first unfinished wait, every wave exits) and the timeout is reported as
+ * SPIRES_ERR_INTERNAL by the call that waits for that run; the flag is then
+ * cleared, and a reset gives a reservoir that runs normally.  No counterpart
+ * in the reference. */
+spires_status spires_ext_inject_exchange_timeout(spires_reservoir *r);
 
 /* Step arithmetic. */
 #define SPIRES_EXT_MODE_EXACT 0   /* FP64, bit-exact with the reference CPU path (default) */

@@ -109,6 +109,7 @@ SIGNATURES = {
     "spires_ext_fit_ridge_records": (C.c_int, [_P, _SZ, _SZ, _D, C.c_double, _D]),
     "spires_ext_batch_workgroups": (C.c_int, [C.POINTER(_P), C.c_int]),
     "spires_ext_kernel_plan": (C.c_int, [_P, C.POINTER(C.c_int)]),
+    "spires_ext_inject_exchange_timeout": (C.c_int, [_P]),
     "spires_ext_timeblock_create": (C.c_int, [C.POINTER(_P), C.c_int, _SZ, C.c_uint, C.POINTER(_P)]),
     "spires_ext_timeblock_run": (C.c_int, [_P, _SZ]),
     "spires_ext_timeblock_launches": (C.c_int, [_P]),

@@ -479,6 +479,8 @@ static int grow_record(spires_dev* g, T** buf, size_t* cap, size_t need, size_t 
         hipError_t e = hipMemcpyAsync(nb, *buf, keep_elems * sizeof(T), hipMemcpyDeviceToDevice, g->stream);
         if (e == hipSuccess) e = hipStreamSynchronize(g->stream);
         if (e != hipSuccess) { dfree(nb); return hip_fail(e, "record copy", __LINE__); }
+    } else if (*buf) {
+        (void)hipStreamSynchronize(g->stream);   // a queued launch may still write the old record
     }
     dfree(*buf);
     *buf = nb;
@@ -513,6 +515,8 @@ const char* spires_dev_last_error(void) { return g_err.c_str(); }
 void spires_dev_destroy(spires_dev* g) {
     if (!g) return;
     if (g->stream) (void)hipStreamSynchronize(g->stream);
+    if (g->abort_host) (void)hipHostFree(g->abort_host);
+    g->abort_host = nullptr;
     readout_release(g);
     dfree(g->W_in); dfree(g->W_out); dfree(g->rp); dfree(g->col); dfree(g->val); dfree(g->WT);
     dfree(g->coef); dfree(g->V); dfree(g->S[0]); dfree(g->S[1]); dfree(g->tprev); dfree(g->ring);
@@ -639,11 +643,13 @@ int spires_dev_create(const spires_dev_desc* d, spires_dev** out) {
 
 int spires_dev_reset_neurons(spires_dev* g, const spires_dev_desc* d) {
     if (!g || !d) return SPIRES_DEV_EINVAL;
+    if (const int rc = settle(g)) return rc;
     return init_neurons(g, d, false);
 }
 
 int spires_dev_step(spires_dev* g, const double* u) {
     if (!g) return SPIRES_DEV_EINVAL;
+    if (const int rc = settle(g)) return rc;
     static const double zero = 0.0;
     (void)zero;
     return launch_step(g, nullptr, u, nullptr, nullptr);
@@ -651,6 +657,7 @@ int spires_dev_step(spires_dev* g, const double* u) {
 
 int spires_dev_upload_inputs(spires_dev* g, const double* u, size_t T) {
     if (!g || (!u && T)) return SPIRES_DEV_EINVAL;
+    if (const int rc = settle(g)) return rc;
     const size_t cnt = T * (size_t)g->din;
     if (T > g->u_T) {
         dfree(g->u_series);
@@ -666,9 +673,11 @@ int spires_dev_upload_inputs(spires_dev* g, const double* u, size_t T) {
 int spires_dev_run(spires_dev* g, size_t t0, size_t T, unsigned flags) {
     if (!g) return SPIRES_DEV_EINVAL;
     if (t0 + T > g->u_T && g->din > 0) { set_error("run beyond uploaded inputs"); return SPIRES_DEV_EINVAL; }
+    if (const int rc0 = settle_ready(g)) return rc0;
     const size_t base = (flags & SPIRES_DEV_REC_APPEND) ? t0 : 0;   // first record row of this run
     const size_t bn = base * (size_t)g->n;
     if (g->mode == SPIRES_DEV_MODE_FAST) {
+        if (const int rcs = settle(g)) return rcs;
         if (!fast_supported(g)) { set_error("fast mode supports FLIF-GL with SIMPLE/PSC sparse synapses only"); return SPIRES_DEV_EINVAL; }
         unsigned needf = flags | ((flags & SPIRES_DEV_REC_OUTPUT) ? SPIRES_DEV_REC_STATE : 0u);
         int rcf = ensure_records(g, base + T, needf, base);
@@ -691,6 +700,8 @@ int spires_dev_run(spires_dev* g, size_t t0, size_t T, unsigned flags) {
     g->ring32_valid = 0;
     const bool persist = g->mode == SPIRES_DEV_MODE_EXACT && g->path != 1 && T > 0 && persist_supported(g);
     if (g->path >= 2 && !persist) { set_error("persistent kernel cannot run this reservoir"); return SPIRES_DEV_EINVAL; }
+    if (!persist)   // the per-step path checks its launches as it goes
+        if (const int rcs = settle(g)) return rcs;
     unsigned need = flags;
     if (persist && (flags & SPIRES_DEV_REC_OUTPUT)) need |= SPIRES_DEV_REC_STATE;   // outputs from the state record
     int rc = ensure_records(g, base + T, need, base);
@@ -724,6 +735,7 @@ int spires_dev_run_batch(spires_dev* const* gs, int count, size_t t0, size_t T, 
     for (int r = 0; r < count; ++r) {
         spires_dev* g = gs[r];
         if (!g) return SPIRES_DEV_EINVAL;
+        if (const int rcs = settle(g)) return rcs;
         if (g->mode != SPIRES_DEV_MODE_EXACT || g->path == 1) { set_error("batched runs are exact-mode persistent runs"); return SPIRES_DEV_EINVAL; }
         if (t0 + T > g->u_T && g->din > 0) { set_error("run beyond uploaded inputs"); return SPIRES_DEV_EINVAL; }
         if (!g->ring64_valid) {
@@ -754,12 +766,14 @@ int spires_dev_batch_workgroups(spires_dev* const* gs, int count) {
 
 int spires_dev_reserve_records(spires_dev* g, size_t rows, unsigned flags) {
     if (!g) return SPIRES_DEV_EINVAL;
+    if (const int rc = settle(g)) return rc;
     return ensure_records(g, rows, flags);
 }
 
 int spires_dev_read_records(spires_dev* g, size_t row0, size_t rows, double* states,
                             uint8_t* raster, double* outputs) {
     if (!g || row0 + rows > g->rec_rows) return SPIRES_DEV_EINVAL;
+    if (const int rc = settle(g)) return rc;
     const size_t n = g->n;
     if (states && g->rec_state)
         SDEV_CK(hipMemcpyAsync(states, g->rec_state + row0 * n, rows * n * 8, hipMemcpyDeviceToHost, g->stream));
@@ -773,6 +787,7 @@ int spires_dev_read_records(spires_dev* g, size_t row0, size_t rows, double* sta
 
 int spires_dev_read_state(spires_dev* g, double* V) {
     if (!g || !V) return SPIRES_DEV_EINVAL;
+    if (const int rc = settle(g)) return rc;
     SDEV_CK(hipMemcpyAsync(V, g->V, (size_t)g->n * 8, hipMemcpyDeviceToHost, g->stream));
     SDEV_CK(hipStreamSynchronize(g->stream));
     return 0;
@@ -780,6 +795,7 @@ int spires_dev_read_state(spires_dev* g, double* V) {
 
 int spires_dev_set_state(spires_dev* g, const double* V) {
     if (!g || !V) return SPIRES_DEV_EINVAL;
+    if (const int rc = settle(g)) return rc;
     SDEV_CK(hipMemcpyAsync(g->V, V, (size_t)g->n * 8, hipMemcpyHostToDevice, g->stream));
     SDEV_CK(hipStreamSynchronize(g->stream));
     return 0;
@@ -787,6 +803,7 @@ int spires_dev_set_state(spires_dev* g, const double* V) {
 
 int spires_dev_read_spikes(spires_dev* g, double* S) {
     if (!g || !S) return SPIRES_DEV_EINVAL;
+    if (const int rc = settle(g)) return rc;
     std::vector<uint8_t> s(g->n);
     SDEV_CK(hipMemcpyAsync(s.data(), g->S[g->cur], (size_t)g->n, hipMemcpyDeviceToHost, g->stream));
     SDEV_CK(hipStreamSynchronize(g->stream));
@@ -796,6 +813,7 @@ int spires_dev_read_spikes(spires_dev* g, double* S) {
 
 int spires_dev_compute_output(spires_dev* g, double* y) {
     if (!g || !y) return SPIRES_DEV_EINVAL;
+    if (const int rc = settle(g)) return rc;
     int rc = launch_output(g, g->ybuf);
     if (rc) return rc;
     if (g->dout) SDEV_CK(hipMemcpyAsync(y, g->ybuf, (size_t)g->dout * 8, hipMemcpyDeviceToHost, g->stream));
@@ -805,6 +823,7 @@ int spires_dev_compute_output(spires_dev* g, double* y) {
 
 int spires_dev_get_readout(spires_dev* g, double* W) {
     if (!g || !W) return SPIRES_DEV_EINVAL;
+    if (const int rc = settle(g)) return rc;
     const size_t cnt = (size_t)g->n * g->dout;
     if (cnt) SDEV_CK(hipMemcpyAsync(W, g->W_out, cnt * 8, hipMemcpyDeviceToHost, g->stream));
     SDEV_CK(hipStreamSynchronize(g->stream));
@@ -813,6 +832,7 @@ int spires_dev_get_readout(spires_dev* g, double* W) {
 
 int spires_dev_set_readout(spires_dev* g, const double* W) {
     if (!g || !W) return SPIRES_DEV_EINVAL;
+    if (const int rc = settle(g)) return rc;
     const size_t cnt = (size_t)g->n * g->dout;
     if (cnt) SDEV_CK(hipMemcpyAsync(g->W_out, W, cnt * 8, hipMemcpyHostToDevice, g->stream));
     SDEV_CK(hipStreamSynchronize(g->stream));
@@ -821,6 +841,7 @@ int spires_dev_set_readout(spires_dev* g, const double* W) {
 
 int spires_dev_set_mode(spires_dev* g, int mode, int block_k) {
     if (!g || (mode != SPIRES_DEV_MODE_EXACT && mode != SPIRES_DEV_MODE_FAST)) return SPIRES_DEV_EINVAL;
+    if (const int rc = settle(g)) return rc;
     g->mode = mode;
     g->block_k = block_k;
     return 0;
@@ -834,6 +855,7 @@ int spires_dev_set_path(spires_dev* g, int path) {
 
 int spires_dev_sync(spires_dev* g) {
     if (!g) return SPIRES_DEV_EINVAL;
+    if (const int rc = settle(g)) return rc;
     SDEV_CK(hipStreamSynchronize(g->stream));
     return 0;
 }

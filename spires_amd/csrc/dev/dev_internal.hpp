@@ -106,7 +106,10 @@ struct spires_dev {
     void* readout_ws = nullptr; // rocBLAS/rocSOLVER handles (readout unit)
     int path = 0;               // 0 auto, 1 per-step kernels, 2 persistent kernel
     unsigned long long* xch = nullptr; size_t xch_words = 0; int xch_plan_G = 0;
-    unsigned* abort_flag = nullptr;
+    unsigned* abort_flag = nullptr;   // raised by a persistent kernel whose exchange wait timed out
+    unsigned* abort_host = nullptr;   // pinned copy of abort_flag after the last asynchronous launch
+    bool abort_pending = false;       // an asynchronous launch's abort flag is still to be checked
+    bool abort_dirty = true;          // abort_flag must be cleared before the next launch
     float* ring32 = nullptr; size_t ring32_cap = 0;   // FP32 history of the fast mode
     int ring32_valid = 0, ring64_valid = 1;
 };
@@ -124,6 +127,13 @@ int batch_plan(spires_dev* const* gs, int count, int* nb_out, int* wgs_out);
 int launch_persist_batch(spires_dev* const* gs, int count, long long t0, long long T, double* const* rec_V,
                          uint8_t* const* rec_S);
 int outputs_from_states(spires_dev* g, const double* states, size_t T, double* out);
+// Waits for g's stream when an asynchronous persistent launch is still to be
+// checked, and reports its exchange timeout (then the flag is cleared before
+// the next launch).  Every entry point that reads results, changes state or
+// launches other kernels on g calls it first.
+int settle(spires_dev* g);
+// The same check without waiting: reports a timeout already copied back.
+int settle_ready(spires_dev* g);
 // Batched FP64 ridge systems of one order n (dev_chol.hip): A [count][n x n]
 // column-major (the Gram fills both triangles; the factorisation keeps L in the
 // lower and L' in the upper one), B / X / R [count][n x D], workspace of

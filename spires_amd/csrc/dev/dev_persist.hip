@@ -1160,7 +1160,10 @@ int prepare_launch(spires_dev* g, const Plan& pl) {
         SDEV_CK(hipMemcpyAsync(g->rbm, bm.data(), bm.size() * 4, hipMemcpyHostToDevice, g->stream));
         SDEV_CK(hipStreamSynchronize(g->stream));
     }
-    SDEV_CK(hipMemsetAsync(g->abort_flag, 0, 4, g->stream));
+    if (g->abort_dirty) {
+        SDEV_CK(hipMemsetAsync(g->abort_flag, 0, 4, g->stream));
+        g->abort_dirty = false;
+    }
     return 0;
 }
 
@@ -1185,6 +1188,8 @@ PersistArgs make_args(spires_dev* g, const Plan& pl, const double* u_series, lon
 }
 
 // After a launch: advance the step counters and report an aborted exchange.
+// The device flag stays raised until the error has been reported, so a launch
+// queued behind an aborted one gives up at its first wait instead of hiding it.
 int finish_launch(spires_dev* g, long long T) {
     g->step += (long long)T * g->micro;
     g->msteps += (size_t)T;
@@ -1192,7 +1197,27 @@ int finish_launch(spires_dev* g, long long T) {
     unsigned flag = 0;
     SDEV_CK(hipMemcpyAsync(&flag, g->abort_flag, 4, hipMemcpyDeviceToHost, g->stream));
     SDEV_CK(hipStreamSynchronize(g->stream));
-    if (flag) { set_error("persistent step kernel timed out waiting for the spike exchange"); return SPIRES_DEV_EINTERNAL; }
+    if (flag) {
+        g->abort_dirty = true;
+        set_error("persistent step kernel timed out waiting for the spike exchange");
+        return SPIRES_DEV_EINTERNAL;
+    }
+    return 0;
+}
+
+// As finish_launch without waiting: the flag is copied to pinned host memory
+// in stream order and checked by the next settle(), so the host can queue the
+// next launch while this one runs.
+int finish_launch_async(spires_dev* g, long long T) {
+    g->step += (long long)T * g->micro;
+    g->msteps += (size_t)T;
+    g->cur ^= 1;
+    if (!g->abort_host) {
+        SDEV_CK(hipHostMalloc((void**)&g->abort_host, sizeof(unsigned), hipHostMallocDefault));
+        *g->abort_host = 0u;
+    }
+    SDEV_CK(hipMemcpyAsync(g->abort_host, g->abort_flag, 4, hipMemcpyDeviceToHost, g->stream));
+    g->abort_pending = true;
     return 0;
 }
 
@@ -1224,7 +1249,26 @@ int launch_persist(spires_dev* g, const double* u_series, long long t0, long lon
     }
     SDEV_CK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)pl.lds));
     SDEV_CK(hipLaunchCooperativeKernel(fn, dim3(pl.G), dim3(threads), args, (unsigned)pl.lds, g->stream));
-    return finish_launch(g, T);
+    return finish_launch_async(g, T);
+}
+
+int settle_ready(spires_dev* g) {
+    if (!g->abort_pending || !g->abort_host) return 0;
+    if (__atomic_load_n(g->abort_host, __ATOMIC_ACQUIRE) == 0u) return 0;
+    return settle(g);
+}
+
+int settle(spires_dev* g) {
+    if (!g || !g->abort_pending) return 0;
+    SDEV_CK(hipStreamSynchronize(g->stream));
+    g->abort_pending = false;
+    if (*g->abort_host) {
+        *g->abort_host = 0u;
+        g->abort_dirty = true;
+        set_error("persistent step kernel timed out waiting for the spike exchange");
+        return SPIRES_DEV_EINTERNAL;
+    }
+    return 0;
 }
 
 int batch_plan(spires_dev* const* gs, int count, int* nb_out, int* wgs_out) {
@@ -1294,6 +1338,24 @@ int launch_persist_batch(spires_dev* const* gs, int count, long long t0, long lo
 }
 
 }  // namespace sdev
+
+// Raises g's exchange abort flag as a timed-out spike-exchange wait would, so
+// that the next persistent launch gives up at its first unfinished wait and the
+// error path can be exercised without a real timeout.
+extern "C" int spires_dev_inject_exchange_timeout(spires_dev* g) {
+    using namespace sdev;
+    if (!g) return SPIRES_DEV_EINVAL;
+    if (const int rc = settle(g)) return rc;
+    if (!g->abort_flag) {
+        const int rc = dalloc(&g->abort_flag, 1);
+        if (rc) return rc;
+    }
+    const unsigned one = 1u;
+    SDEV_CK(hipMemcpyAsync(g->abort_flag, &one, 4, hipMemcpyHostToDevice, g->stream));
+    SDEV_CK(hipStreamSynchronize(g->stream));
+    g->abort_dirty = false;
+    return 0;
+}
 
 extern "C" int spires_dev_plan_info(spires_dev* g, int* info) {
     if (!g || !info) return SPIRES_DEV_EINVAL;
@@ -1384,6 +1446,7 @@ extern "C" int spires_dev_timeblock_create(spires_dev* const* gs, int count, lon
     for (int r = 0; r < count; ++r) {
         spires_dev* g = gs[r];
         if (!g) return SPIRES_DEV_EINVAL;
+        if (const int rc = settle(g)) return rc;
         if (g->mode != SPIRES_DEV_MODE_EXACT || g->path == 1 || !persist_plan(g, ncu, &plans[r], true) || !plans[r].tp) {
             set_error("time blocks need exact-mode FLIF-GL reservoirs that fit the product-wave kernel");
             return SPIRES_DEV_EINVAL;
@@ -1511,6 +1574,10 @@ extern "C" int spires_dev_timeblock_run(spires_dev_timeblock* tb, long long bloc
     const size_t cnt = tb->gs.size();
     const long long steps = blocks * tb->K;
     std::vector<long long> clk(cnt * 4, 0);
+    for (spires_dev* g : tb->gs) {
+        const int rc = settle(g);
+        if (rc) return rc;
+    }
     for (size_t r = 0; r < cnt; ++r) {
         spires_dev* g = tb->gs[r];
         if (g->u_series != tb->u_at[r] || g->rec_state != tb->recv_at[r] || g->rec_raster != tb->recs_at[r]) {

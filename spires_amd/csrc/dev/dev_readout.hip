@@ -296,6 +296,7 @@ int spires_dev_ridge_rows(spires_dev* g, size_t row0, size_t row1, const double*
     hipError_t herr = hipSuccess;
     auto H = [&](hipError_t e) { if (e != hipSuccess && herr == hipSuccess) herr = e; };
     if (!g || !Y) return SPIRES_DEV_EINVAL;
+    if (const int rc0 = settle(g)) return rc0;
     if (!g->rec_state || row0 >= row1 || row1 > g->rec_rows) { set_error("no recorded states for ridge"); return SPIRES_DEV_EINVAL; }
     const size_t washout = row0;
     const int n = g->n, D = g->dout;
@@ -505,6 +506,7 @@ int spires_dev_ridge_rows_batch(spires_dev* const* gs, int count, size_t row0, s
     bool same = true;
     for (int r = 0; r < count; ++r) {
         if (!gs[r] || !Y[r]) return SPIRES_DEV_EINVAL;
+        if (const int rc0 = settle(gs[r])) return rc0;
         if (!gs[r]->rec_state || row0 >= row1 || row1 > gs[r]->rec_rows) { set_error("no recorded states for ridge"); return SPIRES_DEV_EINVAL; }
         same = same && gs[r]->n == n && gs[r]->dout == D && gs[r]->mode == SPIRES_DEV_MODE_EXACT;
     }
@@ -667,6 +669,7 @@ static int ridge_batch_impl(spires_dev* const* gs, int count, size_t row0, size_
 
 int spires_dev_predict_records(spires_dev* g, size_t row0, size_t rows, double* out) {
     if (!g || (!out && rows)) return SPIRES_DEV_EINVAL;
+    if (const int rc0 = settle(g)) return rc0;
     if (!g->rec_state || row0 + rows > g->rec_rows) { set_error("rows beyond the recorded states"); return SPIRES_DEV_EINVAL; }
     if (rows == 0 || g->dout == 0) return 0;
     ReadoutLock lock(g);
@@ -687,6 +690,7 @@ int spires_dev_rls(spires_dev* g, const double* Y, size_t T, double delta, doubl
     hipError_t herr = hipSuccess;
     auto H = [&](hipError_t e) { if (e != hipSuccess && herr == hipSuccess) herr = e; };
     if (!g || (!Y && T)) return SPIRES_DEV_EINVAL;
+    if (const int rc0 = settle(g)) return rc0;
     if (T > g->u_T && g->din > 0) { set_error("rls beyond uploaded inputs"); return SPIRES_DEV_EINVAL; }
     const int n = g->n, D = g->dout;
     const size_t nn = (size_t)n * n;
@@ -721,6 +725,7 @@ int spires_dev_online(spires_dev* g, const double* target, double lr) {
     hipError_t herr = hipSuccess;
     auto H = [&](hipError_t e) { if (e != hipSuccess && herr == hipSuccess) herr = e; };
     if (!g || !target) return SPIRES_DEV_EINVAL;
+    if (const int rc0 = settle(g)) return rc0;
     const int n = g->n, D = g->dout;
     if (D == 0) return 0;
     double* td = nullptr;

@@ -745,6 +745,11 @@ spires_status spires_ext_get_input_weights(const spires_reservoir* r, double* W_
     return SPIRES_OK;
 }
 
+spires_status spires_ext_inject_exchange_timeout(spires_reservoir* r) {
+    if (!r || !r->dev) return SPIRES_ERR_INVALID_ARG;
+    return dev_status(spires_dev_inject_exchange_timeout(r->dev));
+}
+
 const char* spires_ext_last_error(void) { return last_error().c_str(); }
 
 void spires_ext_shutdown(void) { spires_dev_shutdown(); }
