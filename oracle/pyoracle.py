@@ -121,6 +121,10 @@ def lib():
         L.or_lu_solve.argtypes = [_SZ, _D, _D, _D]
         L.or_df_seed.restype = C.c_uint64
         L.or_df_seed.argtypes = [_SZ, _D]
+        L.or_probe_synapse.argtypes = [_P, _D, _SZ, _D]
+        L.or_probe_csr_rows.argtypes = [_SZ, _I64, _I32, _D, _D, _D]
+        L.or_probe_multiexp.restype = C.c_int
+        L.or_probe_multiexp.argtypes = [_P, _D, _D]
         L.or_agile_optimize.restype = C.c_int
         L.or_agile_optimize.argtypes = [_D, _D, _D, C.c_int, LOSS_FN, C.c_void_p,
                                         C.POINTER(AgileOpts), _D, C.POINTER(AgileReport)]
@@ -272,6 +276,32 @@ class OracleReservoir:
         q.N = lib().or_num_neurons(q.h)
         q.Din, q.Dout = self.Din, self.Dout
         return q
+
+
+def probe_synapse(res, spikes):
+    """Vector the row dot reads after each step's synapse_prepare, for a spike
+    train [T x N] (the reservoir's own synapse state advances)."""
+    spikes = np.ascontiguousarray(spikes, np.float64)
+    out = np.empty_like(spikes)
+    lib().or_probe_synapse(res.h, dptr(spikes), spikes.shape[0], dptr(out))
+    return out
+
+
+def probe_csr_rows(rp, ci, val, x):
+    rp = np.ascontiguousarray(rp, np.int64)
+    ci = np.ascontiguousarray(ci, np.int32)
+    val = np.ascontiguousarray(val, np.float64)
+    x = np.ascontiguousarray(x, np.float64)
+    out = np.empty(len(rp) - 1)
+    lib().or_probe_csr_rows(len(rp) - 1, rp.ctypes.data_as(_I64), ci.ctypes.data_as(_I32), dptr(val), dptr(x),
+                            dptr(out))
+    return out
+
+
+def probe_multiexp(res, cap=64):
+    tau, c = np.empty(cap), np.empty(cap)
+    k = lib().or_probe_multiexp(res.h, dptr(tau), dptr(c))
+    return tau[:k], c[:k]
 
 
 def gl_coeffs(alpha, n):

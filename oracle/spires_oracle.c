@@ -948,6 +948,28 @@ or_res *or_coarse_grain(const or_res *r, double thr)
 }
 
 /* ------------------------------------------------------------- accessors */
+/* ---- probes for pinning (tests only) ---- */
+void or_probe_synapse(or_res *r, const double *spikes, size_t T, double *eff_out)
+{
+    size_t n = r->N;
+    for (size_t t = 0; t < T; t++) {
+        synapse_prepare(r, &spikes[t * n]);
+        memcpy(&eff_out[t * n], r->eff, n * sizeof(double));
+    }
+}
+
+void or_probe_csr_rows(size_t n, const int64_t *rp, const int32_t *ci, const double *val,
+                       const double *x, double *out)
+{
+    for (size_t i = 0; i < n; i++) out[i] = csr_dot(rp, ci, val, i, x);
+}
+
+int or_probe_multiexp(const or_res *r, double *tau, double *c)
+{
+    for (int k = 0; k < r->me_N; k++) { tau[k] = r->me_tau[k]; c[k] = r->me_c[k]; }
+    return r->me_N;
+}
+
 size_t or_num_neurons(const or_res *r) { return r ? r->N : 0; }
 size_t or_nnz(const or_res *r) { return r ? r->nnz : 0; }
 void or_get_csr(const or_res *r, int64_t *rp, int32_t *ci, double *val)

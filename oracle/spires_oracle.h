@@ -14,9 +14,13 @@
  *   - neuron updates, GL coefficients, PSC-heterogeneous synapse and AGILE
  *     are checked bit-for-bit against the reference sources compiled from
  *     /root/reference (oracle/_ref, built by oracle/Makefile.ref);
- *   - CSR row dot, power iteration, topologies, LU solve, Caputo weights,
- *     and fractional synapse are checked against closed forms,
- *     brute force, invariants and numpy/scipy library routines.
+ *   - CSR row dot (brute force), PSC-homogeneous, multi-exponential and
+ *     fractional synapses (closed forms, the Mittag-Leffler spectral
+ *     density, a scipy triangular solve of the GL system), RLS (weighted
+ *     ridge by numpy), delta rule (geometric error contraction), power
+ *     iteration (known spectra, eigvalsh), LU and ridge solves (numpy) and
+ *     the construction's rand() draw order (an independent restatement of
+ *     reservoir.c:307-545) are checked in tests/test_oracle_pins.py;
  *   - BLAS-order-dependent results (dense row dot, multi-input W_in*u with a
  *     non power-of-two gain, W_out*V, dgemm Gram, dgesv) are restated in
  *     sequential order; parity with the reference's BLAS build is at
@@ -93,6 +97,18 @@ int    or_lu_solve(size_t n, const double *A, const double *b, double *x);
 /* Diffusive-neuron seed for neuron i (index replaces the reference's heap
  * address; see spires_oracle.c). */
 uint64_t or_df_seed(size_t i, const double *p);
+
+/* ---- probes (tests only) ----
+ * or_probe_synapse: feeds spikes [T x N] to r's synapse_prepare one step at a
+ * time and copies the vector the row dot then reads (trace / current) into
+ * eff_out [T x N] (not meaningful for PSC-heterogeneous, whose traces are per
+ * edge).  or_probe_csr_rows: every row's csr_row_dot over x.
+ * or_probe_multiexp: the multi-exponential synapse's time constants and
+ * mixing weights (returns their count). */
+void or_probe_synapse(or_res *r, const double *spikes, size_t T, double *eff_out);
+void or_probe_csr_rows(size_t n, const int64_t *rp, const int32_t *ci, const double *val,
+                       const double *x, double *out);
+int  or_probe_multiexp(const or_res *r, double *tau, double *c);
 
 /* ---- AGILE (reference src/agile.c) ---- */
 typedef double (*or_loss_fn)(const double *theta, int dim, void *ctx);
