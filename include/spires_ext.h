@@ -190,6 +190,24 @@ const char   *spires_ext_last_error(void);
  * counterpart in the reference, whose library holds no GPU state. */
 void          spires_ext_shutdown(void);
 
+/* One evaluation of spires_optimize's objective, exactly as the optimizer
+ * runs it (eval_adapter + eval_mse, spires_api.c:298-438; the reference keeps
+ * them static): theta[8] = {alpha, spectral radius, input gain,
+ * connectivity, E/I ratio, log10 ridge lambda, N (snapped to 200 / 400 /
+ * 800 / 1200), topology (snapped to 0 / 1 / 2)}.  For each of
+ * budget->num_seeds seeds a reservoir is created with
+ * spires_reservoir_create (consuming rand()), its ridge readout is trained on
+ * the first data_fraction of the series and its MSE measured on the rest
+ * after a reset.  out[5] = {metric mean, metric std (metric = 1 / (1 + MSE)),
+ * cost proxy, feasible (1 / 0), score = mean - lambda_var std - lambda_cost
+ * cost (-1e6 when infeasible)}.  Writes alpha into base->neuron_params[4],
+ * as spires_optimize does. */
+spires_status spires_ext_optimize_evaluate(const spires_reservoir_config *base,
+                                           const struct spires_opt_budget *budget,
+                                           const struct spires_opt_score *score, const double *theta,
+                                           const double *input_series, const double *target_series,
+                                           size_t series_length, double *out);
+
 /* AGILE (Levy exploration + SPSA refinement, the search behind
  * spires_optimize) on a caller-supplied loss.  opts[11] = {mu, eta_min,
  * eta_max, patience0, patience_decay, refine_step, refine_max_iters,

@@ -128,6 +128,8 @@ SIGNATURES = {
     "spires_ext_last_error": (C.c_char_p, []),
     "spires_ext_shutdown": (None, []),
     "spires_ext_agile_optimize": (C.c_int, [_D, _D, _D, C.c_int, LOSS_FN, _P, _D, _D]),
+    "spires_ext_optimize_evaluate": (C.c_int, [C.POINTER(Config), C.POINTER(OptBudget), C.POINTER(OptScore), _D,
+                                               _D, _D, _SZ, _D]),
     "spires_ext_spectral_radius": (C.c_double, [_SZ, _I64, _I32, _D]),
     "spires_ext_mlp_create": (C.c_int, [C.c_int, C.c_int, C.c_ulonglong, C.POINTER(_P)]),
     "spires_ext_mlp_destroy": (None, [_P]),

@@ -252,6 +252,26 @@ extern "C" int spires_ext_agile_optimize(const double* theta0, const double* lo,
     return agile_run(theta0, lo, hi, dim, loss, ctx, o, theta_best);
 }
 
+extern "C" spires_status spires_ext_optimize_evaluate(const spires_reservoir_config* base,
+                                                      const struct spires_opt_budget* budget,
+                                                      const struct spires_opt_score* score, const double* theta,
+                                                      const double* input_series, const double* target_series,
+                                                      size_t series_length, double* out) {
+    if (!base || !budget || !score || !theta || !input_series || !target_series || !out) return SPIRES_ERR_INVALID_ARG;
+    OptCtx ctx{base, input_series, target_series, series_length, budget, score};
+    double t[8];
+    std::memcpy(t, theta, sizeof(t));
+    t[6] = (double)snap_N(t[6]);
+    t[7] = (double)snap_topo(t[7]);
+    const EvalOut e = evaluate(t, &ctx);
+    out[0] = e.mean;
+    out[1] = e.stdv;
+    out[2] = e.cost;
+    out[3] = e.feasible ? 1.0 : 0.0;
+    out[4] = e.feasible ? score_of(e, score) : -1e6;
+    return SPIRES_OK;
+}
+
 extern "C" int spires_optimize(const spires_reservoir_config* base, const struct spires_opt_budget* budgets,
                                int num_budgets, const struct spires_opt_score* score,
                                struct spires_opt_result* out, const double* input_series,
