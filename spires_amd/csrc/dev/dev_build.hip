@@ -259,6 +259,7 @@ extern "C" {
 
 int spires_dev_random_er(int n, double p, double ei, unsigned long long seed, double rho, int64_t** rp_out, int32_t** col_out,
                          double** val_out, size_t* nnz_out, double* lam_out) {
+    sdev::RandShield rand_shield_;
     if (n < 1 || !(p >= 0.0 && p <= 1.0) || !(ei >= 0.0 && ei <= 1.0) || !rp_out || !col_out || !val_out || !nnz_out) return SPIRES_DEV_EINVAL;
     *rp_out = nullptr; *col_out = nullptr; *val_out = nullptr; *nnz_out = 0;
     hipStream_t st = nullptr;
@@ -322,6 +323,7 @@ int spires_dev_random_er(int n, double p, double ei, unsigned long long seed, do
 
 int spires_dev_spectral_radius(int n, const int64_t* rp_h, const int32_t* col_h, const double* val_h, double* lam,
                                int* iters) {
+    sdev::RandShield rand_shield_;
     if (n < 1 || !rp_h || !lam) return SPIRES_DEV_EINVAL;
     const size_t nnz = (size_t)rp_h[n];
     if (nnz && (!col_h || !val_h)) return SPIRES_DEV_EINVAL;
@@ -360,6 +362,7 @@ int spires_dev_spectral_radius(int n, const int64_t* rp_h, const int32_t* col_h,
 
 int spires_dev_spectral_radius_noalloc(int n, const int64_t* rp, const int32_t* col, const double* val,
                                        double* lam, int* iters, void* stream) {
+    sdev::RandShield rand_shield_;
     if (n < 1 || !rp || !col || !val || !lam || !iters) return SPIRES_DEV_EINVAL;
     if (n > kPowerMaxN) {
         set_error("spectral_radius_noalloc: n above 19968 exceeds the fixed work buffers");

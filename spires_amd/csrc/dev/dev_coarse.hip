@@ -162,6 +162,7 @@ extern "C" int spires_dev_coarse_grain(int n, const int64_t* rp_h, const int32_t
                                        double thr, int din, double* W_in, int dout, double* W_out, double* V,
                                        int* alive_h, int* n_out, int64_t** rp_out, int32_t** ci_out,
                                        double** val_out, size_t* nnz_out, int* passes_out) {
+    sdev::RandShield rand_shield_;
     if (n < 1 || !rp_h || !V || !alive_h || !n_out || !rp_out || !ci_out || !val_out || !nnz_out || din < 0 ||
         dout < 0 || (din && !W_in) || (dout && !W_out))
         return SPIRES_DEV_EINVAL;

@@ -13,6 +13,7 @@
 #include "dev_internal.hpp"
 
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <vector>
@@ -21,6 +22,23 @@ namespace sdev {
 
 static thread_local std::string g_err;
 void set_error(const std::string& msg) { g_err = msg; }
+
+namespace {
+std::mutex g_shield_mu;
+int g_shield_depth = 0;
+char* g_shield_prev = nullptr;
+char g_shield_state[256];   // the private generator state (glibc TYPE_4)
+}  // namespace
+
+RandShield::RandShield() {
+    std::lock_guard<std::mutex> lk(g_shield_mu);
+    if (g_shield_depth++ == 0) g_shield_prev = initstate(0x5eedu, g_shield_state, sizeof(g_shield_state));
+}
+
+RandShield::~RandShield() {
+    std::lock_guard<std::mutex> lk(g_shield_mu);
+    if (--g_shield_depth == 0 && g_shield_prev) setstate(g_shield_prev);
+}
 int hip_fail(hipError_t e, const char* what, int line) {
     set_error(std::string(what) + " failed (line " + std::to_string(line) + "): " + hipGetErrorString(e));
     (void)hipGetLastError();
@@ -513,6 +531,7 @@ extern "C" {
 const char* spires_dev_last_error(void) { return g_err.c_str(); }
 
 void spires_dev_destroy(spires_dev* g) {
+    sdev::RandShield rand_shield_;
     if (!g) return;
     if (g->stream) (void)hipStreamSynchronize(g->stream);
     if (g->abort_host) (void)hipHostFree(g->abort_host);
@@ -530,6 +549,7 @@ void spires_dev_destroy(spires_dev* g) {
 }
 
 int spires_dev_create(const spires_dev_desc* d, spires_dev** out) {
+    sdev::RandShield rand_shield_;
     if (!d || !out || d->n <= 0 || d->micro <= 0 || d->din < 0 || d->dout < 0) {
         set_error("invalid descriptor");
         return SPIRES_DEV_EINVAL;
@@ -642,12 +662,14 @@ int spires_dev_create(const spires_dev_desc* d, spires_dev** out) {
 }
 
 int spires_dev_reset_neurons(spires_dev* g, const spires_dev_desc* d) {
+    sdev::RandShield rand_shield_;
     if (!g || !d) return SPIRES_DEV_EINVAL;
     if (const int rc = settle(g)) return rc;
     return init_neurons(g, d, false);
 }
 
 int spires_dev_step(spires_dev* g, const double* u) {
+    sdev::RandShield rand_shield_;
     if (!g) return SPIRES_DEV_EINVAL;
     if (const int rc = settle(g)) return rc;
     static const double zero = 0.0;
@@ -656,6 +678,7 @@ int spires_dev_step(spires_dev* g, const double* u) {
 }
 
 int spires_dev_upload_inputs(spires_dev* g, const double* u, size_t T) {
+    sdev::RandShield rand_shield_;
     if (!g || (!u && T)) return SPIRES_DEV_EINVAL;
     if (const int rc = settle(g)) return rc;
     const size_t cnt = T * (size_t)g->din;
@@ -671,6 +694,7 @@ int spires_dev_upload_inputs(spires_dev* g, const double* u, size_t T) {
 }
 
 int spires_dev_run(spires_dev* g, size_t t0, size_t T, unsigned flags) {
+    sdev::RandShield rand_shield_;
     if (!g) return SPIRES_DEV_EINVAL;
     if (t0 + T > g->u_T && g->din > 0) { set_error("run beyond uploaded inputs"); return SPIRES_DEV_EINVAL; }
     if (const int rc0 = settle_ready(g)) return rc0;
@@ -727,6 +751,7 @@ int spires_dev_run(spires_dev* g, size_t t0, size_t T, unsigned flags) {
 }
 
 int spires_dev_run_batch(spires_dev* const* gs, int count, size_t t0, size_t T, unsigned flags) {
+    sdev::RandShield rand_shield_;
     if (!gs || count < 1) return SPIRES_DEV_EINVAL;
     std::vector<double*> rv(count);
     std::vector<uint8_t*> rs(count);
@@ -759,12 +784,14 @@ int spires_dev_run_batch(spires_dev* const* gs, int count, size_t t0, size_t T, 
 }
 
 int spires_dev_batch_workgroups(spires_dev* const* gs, int count) {
+    sdev::RandShield rand_shield_;
     int total = 0;
     if (!gs || count < 1 || batch_plan(gs, count, nullptr, &total)) return 0;
     return total;
 }
 
 int spires_dev_reserve_records(spires_dev* g, size_t rows, unsigned flags) {
+    sdev::RandShield rand_shield_;
     if (!g) return SPIRES_DEV_EINVAL;
     if (const int rc = settle(g)) return rc;
     return ensure_records(g, rows, flags);
@@ -772,6 +799,7 @@ int spires_dev_reserve_records(spires_dev* g, size_t rows, unsigned flags) {
 
 int spires_dev_read_records(spires_dev* g, size_t row0, size_t rows, double* states,
                             uint8_t* raster, double* outputs) {
+    sdev::RandShield rand_shield_;
     if (!g || row0 + rows > g->rec_rows) return SPIRES_DEV_EINVAL;
     if (const int rc = settle(g)) return rc;
     const size_t n = g->n;
@@ -786,6 +814,7 @@ int spires_dev_read_records(spires_dev* g, size_t row0, size_t rows, double* sta
 }
 
 int spires_dev_read_state(spires_dev* g, double* V) {
+    sdev::RandShield rand_shield_;
     if (!g || !V) return SPIRES_DEV_EINVAL;
     if (const int rc = settle(g)) return rc;
     SDEV_CK(hipMemcpyAsync(V, g->V, (size_t)g->n * 8, hipMemcpyDeviceToHost, g->stream));
@@ -794,6 +823,7 @@ int spires_dev_read_state(spires_dev* g, double* V) {
 }
 
 int spires_dev_set_state(spires_dev* g, const double* V) {
+    sdev::RandShield rand_shield_;
     if (!g || !V) return SPIRES_DEV_EINVAL;
     if (const int rc = settle(g)) return rc;
     SDEV_CK(hipMemcpyAsync(g->V, V, (size_t)g->n * 8, hipMemcpyHostToDevice, g->stream));
@@ -802,6 +832,7 @@ int spires_dev_set_state(spires_dev* g, const double* V) {
 }
 
 int spires_dev_read_spikes(spires_dev* g, double* S) {
+    sdev::RandShield rand_shield_;
     if (!g || !S) return SPIRES_DEV_EINVAL;
     if (const int rc = settle(g)) return rc;
     std::vector<uint8_t> s(g->n);
@@ -812,6 +843,7 @@ int spires_dev_read_spikes(spires_dev* g, double* S) {
 }
 
 int spires_dev_compute_output(spires_dev* g, double* y) {
+    sdev::RandShield rand_shield_;
     if (!g || !y) return SPIRES_DEV_EINVAL;
     if (const int rc = settle(g)) return rc;
     int rc = launch_output(g, g->ybuf);
@@ -822,6 +854,7 @@ int spires_dev_compute_output(spires_dev* g, double* y) {
 }
 
 int spires_dev_get_readout(spires_dev* g, double* W) {
+    sdev::RandShield rand_shield_;
     if (!g || !W) return SPIRES_DEV_EINVAL;
     if (const int rc = settle(g)) return rc;
     const size_t cnt = (size_t)g->n * g->dout;
@@ -831,6 +864,7 @@ int spires_dev_get_readout(spires_dev* g, double* W) {
 }
 
 int spires_dev_set_readout(spires_dev* g, const double* W) {
+    sdev::RandShield rand_shield_;
     if (!g || !W) return SPIRES_DEV_EINVAL;
     if (const int rc = settle(g)) return rc;
     const size_t cnt = (size_t)g->n * g->dout;
@@ -840,6 +874,7 @@ int spires_dev_set_readout(spires_dev* g, const double* W) {
 }
 
 int spires_dev_set_mode(spires_dev* g, int mode, int block_k) {
+    sdev::RandShield rand_shield_;
     if (!g || (mode != SPIRES_DEV_MODE_EXACT && mode != SPIRES_DEV_MODE_FAST)) return SPIRES_DEV_EINVAL;
     if (const int rc = settle(g)) return rc;
     g->mode = mode;
@@ -848,12 +883,14 @@ int spires_dev_set_mode(spires_dev* g, int mode, int block_k) {
 }
 
 int spires_dev_set_path(spires_dev* g, int path) {
+    sdev::RandShield rand_shield_;
     if (!g || path < 0 || path > 4) return SPIRES_DEV_EINVAL;
     g->path = path;
     return 0;
 }
 
 int spires_dev_sync(spires_dev* g) {
+    sdev::RandShield rand_shield_;
     if (!g) return SPIRES_DEV_EINVAL;
     if (const int rc = settle(g)) return rc;
     SDEV_CK(hipStreamSynchronize(g->stream));

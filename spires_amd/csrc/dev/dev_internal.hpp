@@ -17,6 +17,21 @@ enum { BK_SPARSE = 0, BK_DENSE = 1 };
 void set_error(const std::string& msg);
 int hip_fail(hipError_t e, const char* what, int line);
 
+// glibc's rand() sequence belongs to the caller: the reference's construction
+// draws the weights from it (callers seed it with srand()), and the HIP runtime
+// and rocBLAS draw from it on first use.  While a RandShield is alive, rand()
+// reads a private state; the last one to end switches the caller's state back,
+// at the position it had.  Every device-layer entry point holds one, so no GPU
+// call moves the caller's sequence.  (Another thread's rand() calls made while
+// a device call is inside the runtime also read the private state.)
+class RandShield {
+  public:
+    RandShield();
+    ~RandShield();
+    RandShield(const RandShield&) = delete;
+    RandShield& operator=(const RandShield&) = delete;
+};
+
 #define SDEV_CK(x)                                                        \
     do {                                                                  \
         hipError_t e_ = (x);                                              \

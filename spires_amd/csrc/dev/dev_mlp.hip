@@ -19,6 +19,7 @@
 #include <vector>
 
 #include "spires_ext.h"
+#include "dev_internal.hpp"
 
 namespace {
 
@@ -274,6 +275,7 @@ static spires_status mlp_stage(spires_mlp* n, int B) {
 extern "C" {
 
 spires_status spires_ext_mlp_create(int in_dim, int hidden, unsigned long long seed, spires_mlp** out) {
+    sdev::RandShield rand_shield_;
     if (!out || in_dim < 1 || in_dim > kMaxIn || hidden < 1 || hidden > kMaxHidden) {
         mlp_set_error("mlp: in_dim must be 1..16 and hidden 1..64");
         return SPIRES_ERR_INVALID_ARG;
@@ -308,6 +310,7 @@ spires_status spires_ext_mlp_create(int in_dim, int hidden, unsigned long long s
 }
 
 void spires_ext_mlp_destroy(spires_mlp* n) {
+    sdev::RandShield rand_shield_;
     if (!n) return;
     (void)hipFree(n->p); (void)hipFree(n->g); (void)hipFree(n->m); (void)hipFree(n->v); (void)hipFree(n->loss);
     (void)hipFree(n->gpart); (void)hipFree(n->lpart);
@@ -319,6 +322,7 @@ void spires_ext_mlp_destroy(spires_mlp* n) {
 int spires_ext_mlp_num_params(const spires_mlp* n) { return n ? n->sh.count : 0; }
 
 spires_status spires_ext_mlp_get_params(spires_mlp* n, float* params) {
+    sdev::RandShield rand_shield_;
     if (!n || !params) return SPIRES_ERR_INVALID_ARG;
     MLP_CK(hipMemcpyAsync(params, n->p, (size_t)n->sh.count * sizeof(float), hipMemcpyDeviceToHost, n->stream));
     MLP_CK(hipStreamSynchronize(n->stream));
@@ -326,6 +330,7 @@ spires_status spires_ext_mlp_get_params(spires_mlp* n, float* params) {
 }
 
 spires_status spires_ext_mlp_set_params(spires_mlp* n, const float* params) {
+    sdev::RandShield rand_shield_;
     if (!n || !params) return SPIRES_ERR_INVALID_ARG;
     MLP_CK(hipMemcpyAsync(n->p, params, (size_t)n->sh.count * sizeof(float), hipMemcpyHostToDevice, n->stream));
     MLP_CK(hipMemsetAsync(n->m, 0, (size_t)n->sh.count * sizeof(float), n->stream));
@@ -336,6 +341,7 @@ spires_status spires_ext_mlp_set_params(spires_mlp* n, const float* params) {
 }
 
 spires_status spires_ext_mlp_predict(spires_mlp* n, const float* X, int B, float* y) {
+    sdev::RandShield rand_shield_;
     if (!n || !X || !y || B < 0) return SPIRES_ERR_INVALID_ARG;
     if (B == 0) return SPIRES_OK;
     spires_status rc = mlp_stage(n, B);
@@ -351,6 +357,7 @@ spires_status spires_ext_mlp_predict(spires_mlp* n, const float* X, int B, float
 
 spires_status spires_ext_mlp_gradient(spires_mlp* n, const float* X, const float* T, int B, float* grad,
                                       float* loss) {
+    sdev::RandShield rand_shield_;
     if (!n || !X || !T || B < 1) return SPIRES_ERR_INVALID_ARG;
     spires_status rc = mlp_stage(n, B);
     if (rc) return rc;
@@ -367,6 +374,7 @@ spires_status spires_ext_mlp_gradient(spires_mlp* n, const float* X, const float
 
 spires_status spires_ext_mlp_train(spires_mlp* n, const float* X, const float* T, int B, int epochs, float lr,
                                    float* loss) {
+    sdev::RandShield rand_shield_;
     if (!n || !X || !T || B < 1 || epochs < 0) return SPIRES_ERR_INVALID_ARG;
     spires_status rc = mlp_stage(n, B);
     if (rc) return rc;

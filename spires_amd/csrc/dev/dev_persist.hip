@@ -1343,6 +1343,7 @@ int launch_persist_batch(spires_dev* const* gs, int count, long long t0, long lo
 // that the next persistent launch gives up at its first unfinished wait and the
 // error path can be exercised without a real timeout.
 extern "C" int spires_dev_inject_exchange_timeout(spires_dev* g) {
+    sdev::RandShield rand_shield_;
     using namespace sdev;
     if (!g) return SPIRES_DEV_EINVAL;
     if (const int rc = settle(g)) return rc;
@@ -1358,6 +1359,7 @@ extern "C" int spires_dev_inject_exchange_timeout(spires_dev* g) {
 }
 
 extern "C" int spires_dev_plan_info(spires_dev* g, int* info) {
+    sdev::RandShield rand_shield_;
     if (!g || !info) return SPIRES_DEV_EINVAL;
     for (int k = 0; k < 7; ++k) info[k] = 0;
     sdev::Plan pl, pb;
@@ -1423,6 +1425,7 @@ struct spires_dev_timeblock {
 };
 
 extern "C" void spires_dev_timeblock_destroy(spires_dev_timeblock* tb) {
+    sdev::RandShield rand_shield_;
     if (!tb) return;
     if (tb->stream) (void)hipStreamSynchronize(tb->stream);
     if (tb->exec) (void)hipGraphExecDestroy(tb->exec);
@@ -1434,6 +1437,7 @@ extern "C" void spires_dev_timeblock_destroy(spires_dev_timeblock* tb) {
 
 extern "C" int spires_dev_timeblock_create(spires_dev* const* gs, int count, long long K, unsigned flags,
                                            spires_dev_timeblock** out) {
+    sdev::RandShield rand_shield_;
     using namespace sdev;
     if (!gs || count < 1 || K < 1 || !out) return SPIRES_DEV_EINVAL;
     *out = nullptr;
@@ -1568,6 +1572,7 @@ extern "C" int spires_dev_timeblock_create(spires_dev* const* gs, int count, lon
 extern "C" int spires_dev_timeblock_launches(const spires_dev_timeblock* tb) { return tb ? tb->launches : 0; }
 
 extern "C" int spires_dev_timeblock_run(spires_dev_timeblock* tb, long long blocks) {
+    sdev::RandShield rand_shield_;
     using namespace sdev;
     if (!tb || blocks < 0) return SPIRES_DEV_EINVAL;
     if (blocks == 0) return 0;

@@ -48,6 +48,7 @@ static ReadoutCtx* readout_ctx() {
 // Destroys every device's handle and frees its workspace, each with its device
 // current; the process-exit path then finds no rocBLAS or HIP object of ours alive.
 void readout_shutdown() {
+    RandShield rand_shield;
     int cur = 0;
     const bool have_cur = hipGetDevice(&cur) == hipSuccess;
     for (int d = 0; d < kMaxDevices; ++d) {
@@ -287,12 +288,14 @@ static int ridge_batch_impl(spires_dev* const* gs, int count, size_t row0, size_
                             const double* lambda, double* info_out);
 
 int spires_dev_ridge(spires_dev* g, size_t washout, const double* Y, double lambda, double* info_out) {
+    sdev::RandShield rand_shield_;
     if (!g) return SPIRES_DEV_EINVAL;
     return spires_dev_ridge_rows(g, washout, g->rec_rows, Y, lambda, info_out);
 }
 
 int spires_dev_ridge_rows(spires_dev* g, size_t row0, size_t row1, const double* Y, double lambda,
                           double* info_out) {
+    sdev::RandShield rand_shield_;
     hipError_t herr = hipSuccess;
     auto H = [&](hipError_t e) { if (e != hipSuccess && herr == hipSuccess) herr = e; };
     if (!g || !Y) return SPIRES_DEV_EINVAL;
@@ -499,6 +502,7 @@ int spires_dev_ridge_rows(spires_dev* g, size_t row0, size_t row1, const double*
 
 int spires_dev_ridge_rows_batch(spires_dev* const* gs, int count, size_t row0, size_t row1, const double* const* Y,
                                 const double* lambda, double* info_out) {
+    sdev::RandShield rand_shield_;
     if (!gs || count < 1 || !Y || !lambda) return SPIRES_DEV_EINVAL;
     spires_dev* g0 = gs[0];
     if (!g0) return SPIRES_DEV_EINVAL;
@@ -668,6 +672,7 @@ static int ridge_batch_impl(spires_dev* const* gs, int count, size_t row0, size_
 }
 
 int spires_dev_predict_records(spires_dev* g, size_t row0, size_t rows, double* out) {
+    sdev::RandShield rand_shield_;
     if (!g || (!out && rows)) return SPIRES_DEV_EINVAL;
     if (const int rc0 = settle(g)) return rc0;
     if (!g->rec_state || row0 + rows > g->rec_rows) { set_error("rows beyond the recorded states"); return SPIRES_DEV_EINVAL; }
@@ -687,6 +692,7 @@ int spires_dev_predict_records(spires_dev* g, size_t row0, size_t rows, double* 
 }
 
 int spires_dev_rls(spires_dev* g, const double* Y, size_t T, double delta, double lambda) {
+    sdev::RandShield rand_shield_;
     hipError_t herr = hipSuccess;
     auto H = [&](hipError_t e) { if (e != hipSuccess && herr == hipSuccess) herr = e; };
     if (!g || (!Y && T)) return SPIRES_DEV_EINVAL;
@@ -722,6 +728,7 @@ int spires_dev_rls(spires_dev* g, const double* Y, size_t T, double delta, doubl
 }
 
 int spires_dev_online(spires_dev* g, const double* target, double lr) {
+    sdev::RandShield rand_shield_;
     hipError_t herr = hipSuccess;
     auto H = [&](hipError_t e) { if (e != hipSuccess && herr == hipSuccess) herr = e; };
     if (!g || !target) return SPIRES_DEV_EINVAL;
