@@ -328,6 +328,17 @@ def halving_search(ev, thetas, seeds, rungs, target=None, eta=3, batch=32, log=N
     return out
 
 
+def step_cost(th):
+    """Time of one step of a batched launch of four trials like th, in us, as
+    measured for N=2048: a fixed 4.6 us (spike exchange, neuron tail, launch
+    share), the GL history at about 1 us per 200 lags, and the row pull, which
+    grows with the connection probability once rows are dense enough for the
+    column-bitmap pull (0.3 us at p = 0.01, 3 us at 0.05, 4.5-7 us at 0.2)."""
+    th = np.atleast_2d(th)
+    pull = np.maximum(0.3, 3.0 + 2.2 * np.log2(th[:, 3] / 0.05))
+    return 4.6 + th[:, 4] / 200.0 + pull
+
+
 def _levy_steps(rng, count, dim, mu=1.3, lo=1e-3, hi=0.3):
     """Step lengths from a power law p(s) ~ s^-mu truncated to [lo, hi]
     (inverse transform), times random unit directions."""
@@ -342,7 +353,7 @@ def _levy_steps(rng, count, dim, mu=1.3, lo=1e-3, hi=0.3):
 def surrogate_search(ev, rungs, target=None, budget=512, warm=96, wave=32, promote=4, pool=2048, seed=1,
                      epochs=3000, round_epochs=500, lr=1e-3, reseed=6, reseed_top=3, reseed_direct=True,
                      explore=8, top_weight=4, clip=2.0, novelty=0.5, levy_hi=0.15, low_rungs_warm_only=True,
-                     log=None):
+                     cost_weight=0.15, log=None):
     """Surrogate-guided search with the fidelity rungs.
 
     Every trial first runs to the middle rung (the lowest rung is a checkpoint
@@ -364,6 +375,9 @@ def surrogate_search(ev, rungs, target=None, budget=512, warm=96, wave=32, promo
         by predicted score minus `novelty` x distance to the nearest trial run;
       - the rest from its ranking of `pool` Levy-flight moves around the eight
         best top-rung trials.
+    Global and Levy proposals are ranked by predicted log10 NRMSE plus
+    `cost_weight` x log2 of their step cost (step_cost): of two candidates
+    predicted alike, the cheaper one runs.
     With `low_rungs_warm_only`, rungs below the middle one are scored for the
     first wave only (they tell the surrogate how scores change with fidelity);
     later waves run straight to the middle rung.
@@ -446,12 +460,12 @@ def surrogate_search(ev, rungs, target=None, budget=512, warm=96, wave=32, promo
         seen = to_unit(np.array([h[0] for h in history]))
         gz = to_unit(glob)
         dist = np.sqrt(((gz[:, None, :] - seen[None, :, :]) ** 2).sum(axis=2)).min(axis=1)
-        acq = pg - novelty * dist
+        acq = pg - novelty * dist + cost_weight * np.log2(step_cost(glob))
         pick += list(glob[:explore // 2])                                   # unscreened
         pick += [glob[i] for i in np.argsort(acq) if i >= explore // 2][:explore - explore // 2]
         base = to_unit(np.array([d[0] for d in inc[:8]]))
         loc = from_unit(base[rng.integers(len(base), size=pool)] + _levy_steps(rng, pool, 6, hi=levy_hi))
-        pl = net.predict(encode(loc, f_max, f_min, f_max))
+        pl = net.predict(encode(loc, f_max, f_min, f_max)) + cost_weight * np.log2(step_cost(loc))
         pick += [loc[i] for i in np.argsort(pl)[:max(0, wave - len(pick))]]
         run_wave([Trial(th, s) for th, s in zip(pick, seeds(len(pick)))], round_epochs,
                  forced=reseed if reseed_direct else 0, fids=[mid] if low_rungs_warm_only else None)
