@@ -680,6 +680,7 @@ def run_step(args):
     achieved = alg_per_launch / (float(launch_ms.mean()) * 1e-3) / 1e9
     us_step_med = float(np.median(launch_ms)) * 1e3 / block
     floor_us = (L - 1) * ADD_LATENCY_CYCLES / (CLOCK_GHZ * 1e3)
+    xch_us = sa.exchange_floor(plan["workgroups"], 20000)
     traffic_step, traffic_src = measured_traffic_per_step()
     G, _, _ = r.read_records(L, 25, states=True) if T >= L + 25 else (None, None, None)
     X, _, _ = r.read_records(T - 1, 1, states=True)
@@ -714,9 +715,13 @@ def run_step(args):
                      "algorithmic_bytes_per_launch": alg_per_launch,
                      "kernel": "k_persist_tp<%d> (one launch per bench step, %d workgroups)" % (plan["nb"],
                                                                                             plan["workgroups"])},
-        "latency_floor": {"bound": "dependent FP64 add chain of the GL sum",
-                          "floor_us_per_time_step": floor_us, "frac": floor_us / us_step_med,
-                          "basis": "%d lags x %.1f cycles at %.1f GHz" % (L - 1, ADD_LATENCY_CYCLES, CLOCK_GHZ)},
+        "latency_floor": {"bound": "spike exchange among the launch's workgroups (measured), above the GL add chain",
+                          "floor_us_per_time_step": max(floor_us, xch_us), "frac": max(floor_us, xch_us) / us_step_med,
+                          "exchange_us": xch_us,
+                          "exchange_basis": "publish + gather of one tagged word per workgroup per step, %d "
+                                            "workgroups, no other work (spires_ext_exchange_floor)" % plan["workgroups"],
+                          "chain_us": floor_us,
+                          "chain_basis": "%d lags x %.1f cycles at %.1f GHz" % (L - 1, ADD_LATENCY_CYCLES, CLOCK_GHZ)},
         "wall_s": wall,
         "final_state_checksum": float(np.sum(X)),
     }

@@ -121,6 +121,11 @@ int  spires_dev_plan_info(spires_dev *g, int *info);
  * the next persistent launch gives up at its first unfinished wait and the
  * launch reports the timeout (fault injection for error-path tests). */
 int  spires_dev_inject_exchange_timeout(spires_dev *g);
+/* Time per step of the spike exchange alone among G workgroups (one per
+ * compute unit, G <= 512), over T steps of the persistent kernels' publish /
+ * gather protocol with no other work: the floor the exchange puts under a
+ * persistent step. */
+int  spires_dev_exchange_floor(int G, long long T, double *us_per_step);
 /* Pre-size the record buffers (so a later run allocates nothing). */
 int  spires_dev_reserve_records(spires_dev *g, size_t rows, unsigned flags);
 int  spires_dev_read_records(spires_dev *g, size_t row0, size_t rows, double *states,

@@ -164,6 +164,11 @@ spires_status spires_ext_kernel_plan(spires_reservoir *r, int *info);
  * cleared, and a reset gives a reservoir that runs normally.  No counterpart
  * in the reference. */
 spires_status spires_ext_inject_exchange_timeout(spires_reservoir *r);
+/* Measured floor of a persistent step: microseconds per step of the spike
+ * exchange alone among `workgroups` workgroups (at most one per compute unit),
+ * timed over `steps` steps with no other work.  No counterpart in the
+ * reference. */
+spires_status spires_ext_exchange_floor(int workgroups, size_t steps, double *us_per_step);
 
 /* Step arithmetic. */
 #define SPIRES_EXT_MODE_EXACT 0   /* FP64, bit-exact with the reference CPU path (default) */

@@ -110,6 +110,7 @@ SIGNATURES = {
     "spires_ext_batch_workgroups": (C.c_int, [C.POINTER(_P), C.c_int]),
     "spires_ext_kernel_plan": (C.c_int, [_P, C.POINTER(C.c_int)]),
     "spires_ext_inject_exchange_timeout": (C.c_int, [_P]),
+    "spires_ext_exchange_floor": (C.c_int, [C.c_int, _SZ, C.POINTER(C.c_double)]),
     "spires_ext_timeblock_create": (C.c_int, [C.POINTER(_P), C.c_int, _SZ, C.c_uint, C.POINTER(_P)]),
     "spires_ext_timeblock_run": (C.c_int, [_P, _SZ]),
     "spires_ext_timeblock_launches": (C.c_int, [_P]),
@@ -298,6 +299,14 @@ def build_weights(cfg_tuple):
     for p in (rp, ci, val, wi, wo):
         _libc.free(C.cast(p, _P))
     return out
+
+
+def exchange_floor(workgroups, steps=20000):
+    """Microseconds per step of the persistent kernels' spike exchange alone
+    among `workgroups` workgroups (spires_ext_exchange_floor)."""
+    us = C.c_double()
+    _check(lib().spires_ext_exchange_floor(int(workgroups), int(steps), C.byref(us)))
+    return us.value
 
 
 def run_resident_batch(reservoirs, t0, T, flags=None):

@@ -1339,6 +1339,67 @@ int launch_persist_batch(spires_dev* const* gs, int count, long long t0, long lo
 
 }  // namespace sdev
 
+namespace sdev {
+// The spike exchange alone: G workgroups publish and gather their words every
+// step with the product-wave kernel's own protocol (publish_spikes /
+// gather_spikes), and do nothing else.  Its time per step is the floor that
+// exchange puts under a persistent step of G workgroups.
+__global__ void __launch_bounds__(64) k_exchange_floor(unsigned long long* xch, int G, long long T,
+                                                       const uint8_t* S, unsigned* abort_flag) {
+    __shared__ unsigned bits_s[kMaxWordsPerLane * 64];
+    __shared__ int abort_s;
+    if (threadIdx.x == 0) abort_s = 0;
+    __syncthreads();
+    const int lane = threadIdx.x, g = blockIdx.x;
+    for (long long st = 0; st < T; ++st) {
+        if (!gather_spikes(xch, G, 32, 1, G * 32, st, 0, S, bits_s, abort_flag, &abort_s, lane)) break;
+        publish_spikes(xch, G, g, 1, st, lane, 0ull);
+    }
+}
+}  // namespace sdev
+
+extern "C" int spires_dev_exchange_floor(int G, long long T, double* us_per_step) {
+    using namespace sdev;
+    sdev::RandShield rand_shield_;
+    if (G < 1 || G > 64 * kMaxWordsPerLane || T < 2 || !us_per_step) return SPIRES_DEV_EINVAL;
+    if (G > num_cus()) { set_error("exchange floor: more workgroups than compute units"); return SPIRES_DEV_EINVAL; }
+    unsigned long long* xch = nullptr;
+    uint8_t* S = nullptr;
+    unsigned* ab = nullptr;
+    hipStream_t st = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    auto done = [&](int rc) {
+        if (st) (void)hipStreamSynchronize(st);
+        dfree(xch); dfree(S); dfree(ab);
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+        if (st) (void)hipStreamDestroy(st);
+        return rc;
+    };
+    int rc = 0;
+    if ((rc = dalloc(&xch, 2 * (size_t)G)) || (rc = dalloc(&S, (size_t)G * 32)) || (rc = dalloc(&ab, 1))) return done(rc);
+    hipError_t e = hipStreamCreate(&st);
+    if (e == hipSuccess) e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    if (e == hipSuccess) e = hipMemsetAsync(xch, 0, 2 * (size_t)G * 8, st);
+    if (e == hipSuccess) e = hipMemsetAsync(S, 0, (size_t)G * 32, st);
+    if (e == hipSuccess) e = hipMemsetAsync(ab, 0, 4, st);
+    void* args[] = {&xch, &G, &T, &S, &ab};
+    if (e == hipSuccess) e = hipEventRecord(e0, st);
+    if (e == hipSuccess)
+        e = hipLaunchCooperativeKernel((const void*)k_exchange_floor, dim3(G), dim3(64), args, 0, st);
+    if (e == hipSuccess) e = hipEventRecord(e1, st);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    float ms = 0.0f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+    unsigned flag = 0;
+    if (e == hipSuccess) e = hipMemcpy(&flag, ab, 4, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return done(hip_fail(e, "exchange floor", __LINE__));
+    if (flag) { set_error("exchange floor: a wait timed out"); return done(SPIRES_DEV_EINTERNAL); }
+    *us_per_step = (double)ms * 1e3 / (double)T;
+    return done(0);
+}
+
 // Raises g's exchange abort flag as a timed-out spike-exchange wait would, so
 // that the next persistent launch gives up at its first unfinished wait and the
 // error path can be exercised without a real timeout.

@@ -750,6 +750,11 @@ spires_status spires_ext_inject_exchange_timeout(spires_reservoir* r) {
     return dev_status(spires_dev_inject_exchange_timeout(r->dev));
 }
 
+spires_status spires_ext_exchange_floor(int workgroups, size_t steps, double* us_per_step) {
+    if (workgroups < 1 || steps < 2 || !us_per_step) return SPIRES_ERR_INVALID_ARG;
+    return dev_status(spires_dev_exchange_floor(workgroups, (long long)steps, us_per_step));
+}
+
 const char* spires_ext_last_error(void) { return last_error().c_str(); }
 
 void spires_ext_shutdown(void) { spires_dev_shutdown(); }
