@@ -408,12 +408,21 @@ def run_fast(args):
         w, sp = T // 10, (3 * T) // 4
         fits = {}
         for lam in LAMBDAS:
+            t_fit = time.perf_counter()
             info = r.fit_ridge_records(w, sp, y[w:sp].reshape(-1, 1), lam)
+            t_fit = time.perf_counter() - t_fit
             pred = r.predict_records(sp, T - sp)[:, 0]
-            gram = "fp64" if mode == "exact" else ("fp64 (split-bf16 Gram not positive definite)"
-                                                     if info[0] & 2 else "split-bf16")
+            if mode == "exact":
+                gram, solver = "fp64 (matrix-core kernel)", "LU" if info[0] & 1 else "Cholesky"
+            elif info[0] & 4:
+                gram = "split-bf16 (matrix-core kernel)"
+                solver = "CG on the FP64 normal equations, preconditioned by the Cholesky factor of the " \
+                         "split-bf16 Gram" + (" with a diagonal shift" if info[0] & 8 else "")
+            else:
+                gram = "fp64 (split-bf16 Gram not positive definite)" if info[0] & 2 else "split-bf16 (rocBLAS)"
+                solver = "LU" if info[0] & 1 else "Cholesky"
             fits[f"{lam:g}"] = {"val_nrmse": float(np.sqrt(np.mean((pred - y[sp:T]) ** 2)) / np.std(y[sp:T])),
-                                "gram": gram, "solver": "LU" if info[0] & 1 else "Cholesky", "residual": info[1]}
+                                "gram": gram, "solver": solver, "residual": info[1], "fit_seconds": t_fit}
         out[mode] = {"us_per_step": ms * 1e3 / T, "neuron_steps_per_s": n * T / (ms * 1e-3), "readout": fits}
         res[mode] = r
     # raster agreement and first divergence, in row chunks

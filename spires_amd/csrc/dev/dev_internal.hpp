@@ -170,6 +170,11 @@ hipError_t chol_solve_many(hipStream_t st, int count, int n, int D, const double
 // R = B - A X with A the full symmetric (unfactored) matrices.
 hipError_t resid_batched(hipStream_t st, int count, int n, int D, const double* A, size_t strideA, const double* X,
                          const double* B, double* R, size_t strideX, double* ws);
+// Split-BF16 Gram on the bf16 matrix cores (dev_gram.hip): lower triangle of
+// hi'hi + hi'lo + lo'hi + lambda I for the state record X [T x n]; ws holds
+// gram_split_ws_elems(T, n) bf16.  n must be a multiple of 128.
+int gram_split_bf16(hipStream_t st, const double* X, int T, int n, double lambda, double* G, unsigned short* ws);
+size_t gram_split_ws_elems(int T, int n);
 int fast_supported(spires_dev* g);
 int fast_sync_rings(spires_dev* g, bool to_fast);
 int launch_fast(spires_dev* g, const double* u_series, long long t0, long long T, double* rec_V,

@@ -263,6 +263,199 @@ __global__ void k_online(int n, int dout, double lr, const double* t, const doub
 
 static inline unsigned nb(size_t n, unsigned b) { return (unsigned)((n + b - 1) / b); }
 
+// y[:, d] += alpha[d] x[:, d] over n x D column blocks (alpha on the device).
+__global__ void k_col_axpy(int n, int D, const double* __restrict__ alpha, const double* __restrict__ x,
+                           double* __restrict__ y) {
+    const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= (size_t)n * D) return;
+    y[q] = y[q] + alpha[q / n] * x[q];
+}
+
+// p[:, d] = z[:, d] + beta[d] p[:, d].
+__global__ void k_col_xpby(int n, int D, const double* __restrict__ beta, const double* __restrict__ z,
+                           double* __restrict__ p) {
+    const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= (size_t)n * D) return;
+    p[q] = z[q] + beta[q / n] * p[q];
+}
+
+constexpr int kPcgMaxIters = 240;     // over all restarts
+constexpr int kPcgRestarts = 6;
+constexpr double kPcgTol = 1e-12;      // relative residual of the FP64 normal equations
+constexpr int kShiftTries = 10;
+constexpr int kPcgStall = 6;           // iterations without a new lowest residual before CG stops
+
+// Fast-mode ridge when n is a multiple of 128.  The Gram comes from the
+// split-BF16 matrix-core kernel (dev_gram.hip); its Cholesky factor, of
+// G_bf16 + lambda I, or of G_bf16 + (lambda + sigma) I with the smallest shift
+// sigma in a doubling sequence that makes it positive definite (the split
+// Gram is about 2^-16 off per entry, which can exceed a small lambda), serves
+// as the preconditioner of conjugate gradients on the FP64 normal equations
+// (X'X + lambda I) W = X'Y, applied matrix-free from the state record (two
+// FP64 products with X per iteration), one CG per output.  info[0] gets
+// 4 (split-BF16 Gram + CG) + 8 when a shift was needed; info[1] the final
+// relative residual of the FP64 normal equations.  Returns kFastFallback
+// when no shift in the sequence gives a positive definite factor.
+constexpr int kFastFallback = -100;
+static int ridge_fast_split(spires_dev* g, size_t row0, size_t row1, const double* Y, double lambda,
+                            double* info_out) {
+    hipError_t herr = hipSuccess;
+    auto H = [&](hipError_t e) { if (e != hipSuccess && herr == hipSuccess) herr = e; };
+    const int n = g->n, D = g->dout, T = (int)(row1 - row0);
+    ReadoutLock lock(g);
+    rocblas_handle h = lock.h;
+    if (!h) { set_error("rocblas_create_handle failed"); return SPIRES_DEV_EINTERNAL; }
+    const size_t nn = (size_t)n * n, nd = (size_t)n * D;
+    const size_t bf_doubles = (gram_split_ws_elems(T, n) * 2 + 7) / 8 + 32;
+    int rc = lock.reserve(g, 2 * nn + 7 * nd + 2 * (size_t)T * D + bf_doubles + 4 * (size_t)D, (size_t)1);
+    if (rc) return rc;
+    double* A = lock.c->ws;            // factor of G_bf16 + (lambda + sigma) I
+    double* A0 = A + nn;               // G_bf16 + lambda I (lower)
+    double* B = A0 + nn;               // X'Y
+    double* W = B + nd;                // solution
+    double* R = W + nd;                // residual
+    double* Z = R + nd;                // preconditioned residual
+    double* P = Z + nd;                // search direction
+    double* Q = P + nd;                // A_exact P
+    double* Tmp = Q + nd;
+    double* XW = Tmp + nd;             // X v [T x D]
+    double* Yd = XW + (size_t)T * D;   // targets [T x D]
+    double* coef = Yd + (size_t)T * D; // per-output alpha / beta [2 D]
+    unsigned short* bf = reinterpret_cast<unsigned short*>(coef + 4 * (size_t)D);
+    rocblas_int* info = lock.c->iw;
+    const double* Xs = g->rec_state + row0 * (size_t)n;   // column-major n x T (ld n) == X'
+    const double one = 1.0, zero = 0.0;
+    H(hipMemcpyAsync(Yd, Y, (size_t)T * D * 8, hipMemcpyHostToDevice, g->stream));
+    SDEV_RB(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_transpose, n, D, T, &one, Xs, n, Yd, D, &zero,
+                          B, n));
+    if ((rc = gram_split_bf16(g->stream, Xs, T, n, lambda, A0, bf))) return rc;
+    // diagonal scale for the shift sequence
+    std::vector<double> dg(n);
+    H(hipMemcpy2DAsync(dg.data(), 8, A0, (size_t)(n + 1) * 8, 8, (size_t)n, hipMemcpyDeviceToHost, g->stream));
+    H(hipStreamSynchronize(g->stream));
+    double dmax = 0.0;
+    for (double v : dg) dmax = fmax(dmax, fabs(v));
+    // The split Gram is off by about 2^-16 of the largest diagonal entry per entry;
+    // when lambda does not cover that, start the shift there.
+    const double sig0 = ldexp(dmax, -14);
+    double sigma = lambda >= sig0 ? 0.0 : sig0 - lambda;
+    bool pd = false;
+    for (int k = 0; k < kShiftTries && herr == hipSuccess; ++k) {
+        H(hipMemcpyAsync(A, A0, nn * 8, hipMemcpyDeviceToDevice, g->stream));
+        if (sigma > 0.0) hipLaunchKernelGGL(k_add_diag, dim3(nb((size_t)n, 256)), dim3(256), 0, g->stream, n, A, sigma);
+        SDEV_RB(rocsolver_dpotrf(h, rocblas_fill_lower, n, A, n, info));
+        rocblas_int hinfo = 0;
+        H(hipMemcpyAsync(&hinfo, info, sizeof(hinfo), hipMemcpyDeviceToHost, g->stream));
+        H(hipStreamSynchronize(g->stream));
+        if (hinfo == 0) { pd = true; break; }
+        sigma = sigma > 0.0 ? 4.0 * sigma : sig0;
+    }
+    if (herr != hipSuccess) return hip_fail(herr, "split-BF16 ridge", __LINE__);
+    if (!pd) return kFastFallback;
+    auto apply = [&](const double* v, double* out) {   // out = (X'X + lambda I) v, FP64, from the record
+        if (D == 1) {   // one output: two bandwidth-bound matrix-vector products
+            rocblas_dgemv(h, rocblas_operation_transpose, n, T, &one, Xs, n, v, 1, &zero, XW, 1);
+            rocblas_dgemv(h, rocblas_operation_none, n, T, &one, Xs, n, XW, 1, &zero, out, 1);
+        } else {
+            rocblas_dgemm(h, rocblas_operation_transpose, rocblas_operation_none, T, D, n, &one, Xs, n, v, n, &zero,
+                          XW, T);
+            rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, n, D, T, &one, Xs, n, XW, T, &zero, out,
+                          n);
+        }
+        hipLaunchKernelGGL(k_axpby, dim3(nb(nd, 256)), dim3(256), 0, g->stream, nd, lambda, v, out);
+    };
+    // M^-1 = L^-T L^-1 from the explicit inverse of the factor: two triangular
+    // matrix products per application instead of two latency-bound solves
+    SDEV_RB(rocsolver_dtrtri(h, rocblas_fill_lower, rocblas_diagonal_non_unit, n, A, n, info));
+    auto precond = [&](const double* r, double* z) {
+        H(hipMemcpyAsync(z, r, nd * 8, hipMemcpyDeviceToDevice, g->stream));
+        if (D == 1) {
+            rocblas_dtrmv(h, rocblas_fill_lower, rocblas_operation_none, rocblas_diagonal_non_unit, n, A, n, z, 1);
+            rocblas_dtrmv(h, rocblas_fill_lower, rocblas_operation_transpose, rocblas_diagonal_non_unit, n, A, n, z, 1);
+        } else {
+            rocblas_dtrmm(h, rocblas_side_left, rocblas_fill_lower, rocblas_operation_none, rocblas_diagonal_non_unit,
+                          n, D, &one, A, n, z, n, z, n);
+            rocblas_dtrmm(h, rocblas_side_left, rocblas_fill_lower, rocblas_operation_transpose,
+                          rocblas_diagonal_non_unit, n, D, &one, A, n, z, n, z, n);
+        }
+    };
+    std::vector<double> hb(nd), hv(nd), hw(nd), dots(D), rho(D), bn(D);
+    auto col_dots = [&](const double* a, const double* b, std::vector<double>& out) {   // host, fixed order
+        H(hipMemcpyAsync(hv.data(), a, nd * 8, hipMemcpyDeviceToHost, g->stream));
+        H(hipMemcpyAsync(hw.data(), b, nd * 8, hipMemcpyDeviceToHost, g->stream));
+        H(hipStreamSynchronize(g->stream));
+        for (int d = 0; d < D; ++d) {
+            double s0 = 0.0;
+            for (int i = 0; i < n; ++i) s0 += hv[(size_t)d * n + i] * hw[(size_t)d * n + i];
+            out[d] = s0;
+        }
+    };
+    col_dots(B, B, bn);
+    std::vector<double> rr(D), al(D), be(D);
+    auto residual_rel = [&]() {
+        col_dots(R, R, rr);
+        double m = 0.0;
+        for (int d = 0; d < D; ++d) if (bn[d] > 0) m = fmax(m, sqrt(rr[d] / bn[d]));
+        return m;
+    };
+    // (Re)start from W: R = B - A W exactly, Z = M^-1 R, P = Z.
+    auto restart = [&]() {
+        apply(W, Q);
+        H(hipMemcpyAsync(R, B, nd * 8, hipMemcpyDeviceToDevice, g->stream));
+        hipLaunchKernelGGL(k_axpby, dim3(nb(nd, 256)), dim3(256), 0, g->stream, nd, -1.0, Q, R);
+        precond(R, Z);
+        H(hipMemcpyAsync(P, Z, nd * 8, hipMemcpyDeviceToDevice, g->stream));
+        col_dots(R, Z, rho);
+        return residual_rel();
+    };
+    precond(B, W);                                      // W0 = M^-1 B
+    double rel = restart();
+    double best = rel;
+    H(hipMemcpyAsync(Tmp, W, nd * 8, hipMemcpyDeviceToDevice, g->stream));   // best iterate so far
+    // CG loses orthogonality in floating point once the residual is small next to
+    // the conditioning of the preconditioned system; when it stops improving it is
+    // restarted from its best iterate (recomputing the true residual)
+    int iters = 0;
+    for (int cycle = 0; cycle <= kPcgRestarts && best > kPcgTol && herr == hipSuccess; ++cycle) {
+        if (cycle > 0) {
+            H(hipMemcpyAsync(W, Tmp, nd * 8, hipMemcpyDeviceToDevice, g->stream));
+            rel = restart();
+        }
+        int best_it = -1;
+        for (int it = 0; iters < kPcgMaxIters && rel > kPcgTol && herr == hipSuccess; ++it, ++iters) {
+            apply(P, Q);
+            col_dots(P, Q, dots);
+            for (int d = 0; d < D; ++d) al[d] = dots[d] != 0.0 ? rho[d] / dots[d] : 0.0;
+            H(hipMemcpyAsync(coef, al.data(), D * 8, hipMemcpyHostToDevice, g->stream));
+            hipLaunchKernelGGL(k_col_axpy, dim3(nb(nd, 256)), dim3(256), 0, g->stream, n, D, coef, P, W);
+            for (int d = 0; d < D; ++d) al[d] = -al[d];
+            H(hipMemcpyAsync(coef + D, al.data(), D * 8, hipMemcpyHostToDevice, g->stream));
+            hipLaunchKernelGGL(k_col_axpy, dim3(nb(nd, 256)), dim3(256), 0, g->stream, n, D, coef + D, Q, R);
+            rel = residual_rel();
+            if (rel < best) {
+                best = rel;
+                best_it = it;
+                H(hipMemcpyAsync(Tmp, W, nd * 8, hipMemcpyDeviceToDevice, g->stream));
+            }
+            if (rel <= kPcgTol || it - best_it >= kPcgStall) break;   // converged, or no progress lately
+            precond(R, Z);
+            col_dots(R, Z, dots);
+            for (int d = 0; d < D; ++d) { be[d] = rho[d] != 0.0 ? dots[d] / rho[d] : 0.0; rho[d] = dots[d]; }
+            H(hipMemcpyAsync(coef + 2 * D, be.data(), D * 8, hipMemcpyHostToDevice, g->stream));
+            hipLaunchKernelGGL(k_col_xpby, dim3(nb(nd, 256)), dim3(256), 0, g->stream, n, D, coef + 2 * D, Z, P);
+        }
+        if (best_it < 0) break;                          // this cycle found nothing better
+    }
+    if (info_out) {
+        info_out[0] = 4.0 + (sigma > 0.0 ? 8.0 : 0.0);
+        info_out[1] = best;
+    }
+    H(hipMemcpyAsync(g->W_out, Tmp, nd * 8, hipMemcpyDeviceToDevice, g->stream));
+    H(hipStreamSynchronize(g->stream));
+    if (herr != hipSuccess) return hip_fail(herr, "split-BF16 ridge", __LINE__);
+    return 0;
+}
+
 // Y [T x Dout] = X [T x N] W_out' for a recorded state matrix (row-major).
 int outputs_from_states(spires_dev* g, const double* states, size_t T, double* out) {
     if (g->dout == 0 || T == 0) return 0;
@@ -318,6 +511,10 @@ int spires_dev_ridge_rows(spires_dev* g, size_t row0, size_t row1, const double*
             return SPIRES_DEV_ESINGULAR;
         }
         return 0;
+    }
+    if (g->mode == SPIRES_DEV_MODE_FAST && n % 128 == 0) {   // split-BF16 Gram kernel + preconditioned CG
+        const int rcf = ridge_fast_split(g, row0, row1, Y, lambda, info_out);
+        if (rcf != kFastFallback) return rcf;
     }
     ReadoutLock lock(g);
     rocblas_handle h = lock.h;
