@@ -543,7 +543,7 @@ void spires_dev_destroy(spires_dev* g) {
     dfree(g->edge_decay); dfree(g->edge_tr); dfree(g->frac_coef); dfree(g->frac_hist);
     dfree(g->frac_I); dfree(g->me_tr); dfree(g->me_comb); dfree(g->u_series);
     dfree(g->rec_state); dfree(g->rec_raster); dfree(g->rec_out); dfree(g->ybuf); dfree(g->ustep);
-    dfree(g->xch); dfree(g->abort_flag); dfree(g->ring32); dfree(g->rbm);
+    dfree(g->xch); dfree(g->abort_flag); dfree(g->ring32); dfree(g->rbm); dfree(g->fseg);
     if (g->stream) (void)hipStreamDestroy(g->stream);
     delete g;
 }

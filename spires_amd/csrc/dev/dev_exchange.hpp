@@ -27,18 +27,20 @@ __device__ __forceinline__ void publish_spikes(unsigned long long* xch, int nw, 
 
 // Spike bits of micro-step st-1 into bits_s[nw] (one wave).  At the launch's
 // first step they come from the u8 spike vector S left in HBM by the previous
-// launch.  Returns false (after raising the abort flags) on timeout or when
-// another workgroup aborted.
+// launch; neuron r of workgroup gg is gg * nb + r, or gg + ileave * r when the
+// workgroups interleave neurons (ileave = workgroup count).  Returns false
+// (after raising the abort flags) on timeout or when another workgroup aborted.
 __device__ __forceinline__ bool gather_spikes(const unsigned long long* xch, int nw, int nb, int wpg, int n,
                                               long long st, long long st0, const uint8_t* S, unsigned* bits_s,
-                                              unsigned* abort_flag, int* abort_s, int lane) {
+                                              unsigned* abort_flag, int* abort_s, int lane, int ileave = 0) {
     if (st == st0) {
         for (int w = lane; w < nw; w += 64) {
             const int gg = w / wpg, part = w % wpg;
             unsigned b = 0;
             for (int j = 0; j < 32; ++j) {
-                const int nj = gg * nb + part * 32 + j;
-                if (part * 32 + j < nb && nj < n && S[nj]) b |= 1u << j;
+                const int r = part * 32 + j;
+                const int nj = ileave ? gg + ileave * r : gg * nb + r;
+                if (r < nb && nj < n && S[nj]) b |= 1u << j;
             }
             bits_s[w] = b;
         }

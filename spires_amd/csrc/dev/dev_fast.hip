@@ -8,10 +8,36 @@
 // workgroup's FP32 voltage history (ring [L][N] in HBM, slot = time mod L)
 // with a Toeplitz coefficient matrix B[s][j] = c_{m(s)+j}, m(s) = (n0-s) mod L,
 // generated on the fly from the coefficients in LDS.  Four waves compute it
-// with v_mfma_f32_16x16x4f32 at each block start; then the K steps run like
-// the exact persistent kernel (spike exchange + CSR pull on wave 1) with only
-// the j-term triangle summed step-wise from LDS.  The membrane update,
-// threshold comparison and reset stay FP64 (V = pow(dt,alpha) rhs - H).
+// with v_mfma_f32_16x16x4f32 at each block start (below); then the K steps run like
+// the exact persistent kernel (spike exchange on wave 1) with only the j-term
+// triangle summed step-wise from LDS.  The membrane update, threshold
+// comparison and reset stay FP64 (V = pow(dt,alpha) rhs - H).
+//
+// Neurons are interleaved over the G workgroups: neuron i belongs to workgroup
+// i mod G as its lane i div G.  The hubs of a scale-free graph (the lowest
+// neuron ids) are thereby spread over the workgroups instead of loading one
+// with most of the edges.  The FP32 ring is kept in workgroup order ([L][64 G],
+// column 64 g + r for neuron g + G r), so that a workgroup's history columns
+// are contiguous; the conversions from and to the FP64 ring apply the order.
+//
+// History contraction: each wave sums a quarter of the ring's slots for all 64
+// neurons (float4 loads, eight in flight per lane), and the four partial sums
+// are added into O in wave order, so the result does not depend on timing.
+//
+// In-block history: H[n0 + j] = O[j] + c_1 Vb[j-1] + T_j with
+// T_j = sum_{k=2..j} c_k Vb[j-k].  T_{j+1} needs only voltages up to step j - 1,
+// so waves 2 and 3 (one half of the k range each) compute it during step j's
+// row pull, into a double-buffered LDS array; wave 0 adds the three terms at the
+// update.
+//
+// Row pull: each row's CSR range is cut into segments of kSegEdges edges; the
+// lanes of waves 0 and 1 sum the segments (each in CSR order, from +0.0) after the
+// exchange, and the row's lane adds its segments' partial sums in segment
+// order.  A row of at most kSegEdges edges is one segment, summed exactly as
+// the exact kernel sums it; a hub row of a scale-free graph (hundreds of
+// in-edges) no longer serialises the pull on one lane.  In LDS the edges are
+// stored segment-interleaved (edge q of segment sg at q * seg_max + sg), so the
+// lanes of a wave, which take consecutive segments, read consecutive words.
 #include "dev_internal.hpp"
 #include "dev_exchange.hpp"
 
@@ -22,12 +48,15 @@ namespace sdev {
 
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 
-constexpr int kFastThreads = 256;     // 4 waves: GEMM on all, steps on waves 0-1
+constexpr int kFastThreads = 256;     // 4 waves: contraction on all; per step wave 1 exchanges,
+                                      // waves 0-1 pull, waves 2-3 sum the triangle, wave 0 updates
 constexpr int kFastNb = 64;           // neurons per workgroup (one wave of lanes)
+constexpr int kSegEdges = 8;          // edges per row-pull segment (the pull loads 8 slots)
+constexpr int kOStride = kFastNb + 1; // row stride of O in LDS (padded: conflict-free column writes)
 
 struct FastArgs {
     int n, din, L, nb, G, wpg, micro, K, stype;
-    int ne_lds;                   // LDS capacity for CSR edges per workgroup (0: CSR from HBM)
+    int ne_lds;                   // CSR edge slots in LDS, kSegEdges * seg_max (0: CSR from HBM)
     long long st0, t0, T;
     double gain, dt;
     double np[16];
@@ -36,12 +65,55 @@ struct FastArgs {
     const int64_t* rp; const int32_t* col; const double* val;
     const double* edge_decay; double* edge_tr;
     double* V; const uint8_t* S; uint8_t* S_out; double* tprev;
-    float* ring;                  // [L][n] FP32
+    float* ring;                  // [L][ldr] FP32, workgroup order
+    int ldr;                      // ring row length, 64 G
     const double* coef;           // FP64 coefficients (converted to FP32 in LDS)
     unsigned long long* xch;
     unsigned* abort_flag;
     double* rec_V; uint8_t* rec_S;
+    // segments: [G + 1] first segment of each workgroup, [G][nb + 1] first segment
+    // of each row (relative to its workgroup's first), [segments][2] CSR edge range
+    const int* seg;
+    int seg_max;
 };
+
+// Exchange word and bit of column j when neuron j is lane j / G of workgroup j mod G.
+__device__ __forceinline__ unsigned encode_col_ileave(int j, int G, int wpg) {
+    const int gg = j % G, r = j / G;
+    return ((unsigned)(gg * wpg + (r >> 5)) << 5) | (unsigned)(r & 31);
+}
+
+// The same column as (byte offset of its exchange word << 5) | bit: the LDS
+// address of the word (the spike words start the dynamic LDS) is code >> 5,
+// and v_bfe takes the bit from the low five bits of the code itself.
+__device__ __forceinline__ unsigned encode_col_lds(int j, int G, int wpg) {
+    const int gg = j % G, r = j / G;
+    return ((unsigned)(gg * wpg + (r >> 5)) << 7) | (unsigned)(r & 31);
+}
+
+// Row pull from the CSR in HBM (segment tables too large for the LDS): edges
+// [b, e) in CSR order, as row_pull does for the natural neuron order.
+__device__ __forceinline__ double row_pull_ileave(int stype, int b, int e, const int32_t* __restrict__ col,
+                                                  const double* __restrict__ val, const unsigned* __restrict__ bits,
+                                                  int G, int wpg, const double* __restrict__ edge_decay,
+                                                  double* __restrict__ edge_tr) {
+    double s = 0.0;
+    for (int q = b; q < e; ++q) {
+        const unsigned pk = encode_col_ileave(col[q], G, wpg);
+        const unsigned sbit = (bits[pk >> 5] >> (pk & 31)) & 1u;
+        if (stype == ST_SIMPLE) {
+            s = s + (sbit ? val[q] : 0.0);
+        } else {
+            const double d = edge_decay[q];
+            const double tr = edge_tr[q] * d + (1.0 - d) * (double)sbit;
+            edge_tr[q] = tr;
+            s = s + val[q] * tr;
+        }
+    }
+    return s;
+}
+
+static_assert(kSegEdges == 8, "the row pull loads a segment as 8 slots");
 
 __global__ void __launch_bounds__(kFastThreads) k_fast(FastArgs a) {
     extern __shared__ __align__(16) float flds[];   // 16-byte aligned: b128 LDS reads
@@ -49,118 +121,260 @@ __global__ void __launch_bounds__(kFastThreads) k_fast(FastArgs a) {
     // treats them as wave-uniform (scalar branches, scalar loop counters).
     const int g = blockIdx.x, tid = threadIdx.x, lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-    const int nb = a.nb, i0 = g * nb, nloc = min(nb, a.n - i0), L = a.L, K = a.K;
+    const int nb = a.nb, L = a.L, K = a.K;
+    const int nloc = (a.n - g + a.G - 1) / a.G;   // lanes r with g + G r < n
     const int wpg = a.wpg, nw = a.G * wpg;
-    // LDS: coefficients (FP32, padded to even length), pre-block contraction O [K][nb],
-    // in-block voltages Vb [K][nb], then 8-byte data: rec [2][nb], ext [2][nb],
-    // CSR values [ne_lds], then 4-byte data: spike bits [nw], CSR columns [ne_lds]
-    float* c_s = flds;
-    float* O_s = c_s + (L + (L & 1));
-    float* Vb_s = O_s + K * nb;
-    double* rec_s = (double*)(Vb_s + K * nb);
-    double* ext_s = rec_s + 2 * nb;
+    // LDS (each part 16-byte aligned): spike bits [nw] (first, at a fixed address),
+    // coefficients [L] (FP32), pre-block contraction O [K][nb + 1], in-block voltages
+    // Vb [K][nb], triangle partial sums T [2][2][nb], segment partial sums
+    // [seg_max + 8] (FP64), ext [2][nb] (FP64), CSR values [seg_max][kSegEdges]
+    // (FP64), CSR columns [seg_max][kSegEdges] (encode_col_lds), segment ranges
+    // [seg_max][2], row segment offsets [nb + 1]
+    const int nwp = (nw + 3) & ~3, Lp = (L + 3) & ~3;
+    unsigned* bits_s = (unsigned*)flds;
+    float* c_s = flds + nwp;
+    float* O_s = c_s + Lp;
+    float* Vb_s = O_s + ((K * kOStride + 3) & ~3);
+    float* tri_s = Vb_s + K * nb;
+    double* part_s = (double*)(tri_s + 4 * nb);
+    double* ext_s = part_s + ((a.seg_max + 8 + 1) & ~1);
     double* eval_s = ext_s + 2 * nb;
-    unsigned* bits_s = (unsigned*)(eval_s + a.ne_lds);
-    unsigned* eidx_s = bits_s + nw;
+    unsigned* eidx_s = (unsigned*)(eval_s + a.ne_lds);
+    int* segr_s = (int*)(eidx_s + a.ne_lds);
+    int* rowf_s = segr_s + 2 * a.seg_max;
     __shared__ int abort_s;
     if (tid == 0) abort_s = 0;
     for (int k = tid; k < L; k += kFastThreads) c_s[k] = (float)a.coef[k];
-    const int64_t E0 = a.rp[i0];
-    const int ne = (int)(a.rp[i0 + nloc] - E0);
     const bool csr_lds = a.ne_lds > 0;
+    const int sg0 = a.seg[g], nseg = a.seg[g + 1] - sg0;
+    const int* segr = a.seg + (a.G + 1) + (size_t)a.G * (nb + 1) + 2 * (size_t)sg0;
+    for (int e = tid; e < 2 * nseg; e += kFastThreads) segr_s[e] = segr[e];
+    // every segment owns kSegEdges slots; the slots past its length hold column 0
+    // with weight 0 so that the pull loads them unconditionally
     if (csr_lds)
-        for (int e = tid; e < ne; e += kFastThreads) {
-            eidx_s[e] = encode_col(a.col[E0 + e], nb, wpg);
-            eval_s[e] = a.val[E0 + e];
+        for (int sg = tid; sg < nseg; sg += kFastThreads) {
+            const int b = segr[2 * sg], len = segr[2 * sg + 1] - b;
+            for (int q = 0; q < kSegEdges; ++q) {
+                eidx_s[sg * kSegEdges + q] = q < len ? encode_col_lds(a.col[b + q], a.G, wpg) : 0u;
+                eval_s[sg * kSegEdges + q] = q < len ? a.val[b + q] : 0.0;
+            }
         }
+    for (int e = tid; e < a.seg_max + 8; e += kFastThreads) part_s[e] = 0.0;
+    for (int e = tid; e <= nb; e += kFastThreads) rowf_s[e] = a.seg[(a.G + 1) + (size_t)g * (nb + 1) + e];
     const bool own = wave == 0 && lane < nloc;
-    const int i = i0 + lane;
+    const int i = g + a.G * lane;
     double V = own ? a.V[i] : 0.0;
     double tp = own ? a.tprev[i] : 0.0;
     bool spk = own ? (a.S[i] != 0) : false;
-    int rs = 0, re = 0;
     constexpr int kMaxDinReg = 16;
     double win[kMaxDinReg];
     const bool win_reg = a.din <= kMaxDinReg;
 #pragma unroll
     for (int j = 0; j < kMaxDinReg; ++j) win[j] = 0.0;
     if (wave == 1 && lane < nloc) {
-        rs = (int)(a.rp[i] - E0);
-        re = (int)(a.rp[i + 1] - E0);
         if (win_reg)
             for (int j = 0; j < a.din; ++j) win[j] = a.W_in[(size_t)i * a.din + j];
     }
     const double* p = a.np;
+    float vb_last = 0.f;   // wave 0: FP32 voltage of the previous step of the block
     __syncthreads();
 
     long long st = a.st0;
     long long t = 0;
     int mi = 0;
+    int head = (int)(st % L);   // ring slot of micro-step st, advanced without division
     while (t < a.T) {
         // ---------------- block start: O = Hist . B on MFMA (all 4 waves) ----------------
         const long long n0 = st;
         {
-            // wave w owns rows [16w, 16w+16) of the nb = 64 neurons, all K columns (K/16 tiles)
-            const int row = 16 * wave + (lane & 15);
-            const int kq = lane >> 4;                       // 0..3: k within the 4-deep MFMA step
-            floatx4 acc[4];
+            // Wave w sums the history slots of its quarter of the ring, all nb = 64
+            // neurons and all K columns.  Lane l reads slot s = 4 g + l / 16 of slot
+            // group g and the four neurons 4 (l % 16) + c (one float4); MFMA c takes
+            // component c, so its row i is neuron 4 i + c and its k is the slot.
+            // The four partial sums are added into O in wave order.
+            const int kq = lane >> 4, li = lane & 15;
+            floatx4 acc[4][4];   // [c][q]
 #pragma unroll
-            for (int q = 0; q < 4; ++q) acc[q] = floatx4{0.f, 0.f, 0.f, 0.f};
+            for (int c = 0; c < 4; ++c)
+#pragma unroll
+                for (int q = 0; q < 4; ++q) acc[c][q] = floatx4{0.f, 0.f, 0.f, 0.f};
             const int ntile = K / 16;
-            const int gi = i0 + row;
-            for (int s0 = 0; s0 < L; s0 += 4) {
-                const int s = s0 + kq;
-                float av = 0.f;
-                long long mm = 0;
-                if (s < L) {
-                    av = (gi < a.n) ? a.ring[(size_t)s * a.n + gi] : 0.f;
-                    mm = (n0 - s) % L;
+            // ring slot s holds V[n0 - mm], mm = (n0 - s) mod L steps before the block
+            const int nm = head;
+            const int ng = (L + 3) / 4, gpw = (ng + 3) / 4;
+            const int g_lo = wave * gpw, g_hi = min(ng, g_lo + gpw);
+            // loads are unconditional, the slot clamped in bounds; what a clamped load
+            // brings is multiplied by a zero coefficient.  Columns of lanes without a
+            // neuron hold zeros and land in rows of O that are never read.
+            const float* ring_g = a.ring + (size_t)g * kFastNb + 4 * li;   // this lane's 4 columns
+            constexpr int kBatch = 8;   // float4 ring loads in flight before their MFMAs
+            for (int g0 = g_lo; g0 < g_hi; g0 += kBatch) {
+                floatx4 av[kBatch];
+                int mmv[kBatch];
+#pragma unroll
+                for (int b = 0; b < kBatch; ++b) {
+                    const int s = 4 * (g0 + b) + kq;
+                    const bool ok = g0 + b < g_hi && s < L;
+                    av[b] = *reinterpret_cast<const floatx4*>(ring_g + (size_t)min(s, L - 1) * a.ldr);
+                    int mm = nm - s;
                     if (mm < 0) mm += L;
+                    mmv[b] = (ok && mm >= 1 && (long long)mm <= n0) ? mm : -(1 << 30);   // no coefficient
                 }
 #pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    if (q < ntile) {
-                        const int j = 16 * q + (lane & 15);
-                        const long long idx = mm + j;
-                        float bv = 0.f;
-                        if (s < L && mm >= 1 && mm <= n0 && idx <= L - 1) bv = c_s[idx];
-                        acc[q] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv, acc[q], 0, 0, 0);
+                for (int b = 0; b < kBatch; ++b) {
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        if (q < ntile) {
+                            const int idx = mmv[b] + 16 * q + li;
+                            const float cv = c_s[min(max(idx, 0), L - 1)];
+                            const float bv = (idx >= 1 && idx <= L - 1) ? cv : 0.f;
+#pragma unroll
+                            for (int c = 0; c < 4; ++c)
+                                acc[c][q] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[b][c], bv, acc[c][q], 0, 0, 0);
+                        }
                     }
                 }
             }
-            // D layout: lane l, reg r -> row 4*(l/16)+r of the tile, column l%16
+            // D layout: lane l, reg r -> row 4 (l / 16) + r (neuron 4 row + c), column l % 16
+            for (int w2 = 0; w2 < 4; ++w2) {
+                if (wave == w2) {
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                if (q < ntile) {
+                    for (int c = 0; c < 4; ++c)
 #pragma unroll
-                    for (int r = 0; r < 4; ++r) {
-                        const int rr = 16 * wave + 4 * (lane >> 4) + r;
-                        const int cc = 16 * q + (lane & 15);
-                        O_s[cc * nb + rr] = acc[q][r];
-                    }
+                        for (int q = 0; q < 4; ++q) {
+                            if (q < ntile) {
+#pragma unroll
+                                for (int r = 0; r < 4; ++r) {
+                                    const int nl = 4 * (4 * kq + r) + c;
+                                    float* o = O_s + (16 * q + li) * kOStride + nl;
+                                    *o = (w2 == 0) ? acc[c][q][r] : *o + acc[c][q][r];
+                                }
+                            }
+                        }
                 }
+                __syncthreads();
             }
         }
-        __syncthreads();
         // ---------------- K steps (one workgroup barrier each) ----------------
         for (int j = 0; j < K && t < a.T; ++j) {
             const long long tt = a.t0 + t;
-            const int head = (int)(st % L);
-            const int sb = (int)(st & 1), tb = (int)(t & 1);
-            if (wave == 0) {
-                double H = 0.0;
-                if (own && tp > p[6]) {
-                    float h = O_s[j * nb + lane];
-                    for (int k = 1; k <= j; ++k) h = h + c_s[k] * Vb_s[(j - k) * nb + lane];
-                    H = (double)h;
+            const int tb = (int)(t & 1);
+            // wave 0: the history term H and the leak of the update, which need only
+            // the previous steps (T_j was summed during the previous step's pull).
+            // Wave 1: spikes of the previous step and the input drive.
+            double H = 0.0, leak = 0.0;
+            if (wave == 0 && own && tp > p[6]) {
+                float h = O_s[j * kOStride + lane];
+                if (j >= 1) h = h + c_s[1] * vb_last;
+                if (j >= 2) {
+                    h = h + tri_s[(j & 1) * 2 * nb + lane];
+                    h = h + tri_s[(j & 1) * 2 * nb + nb + lane];
                 }
-                __syncthreads();
-                if (__builtin_amdgcn_readfirstlane(abort_s)) break;
+                H = (double)h;
+                leak = -(V - p[2]) / p[3];
+            }
+            if (wave == 1) {
+                double uin[kMaxDinReg];
+                if (mi == 0 && win_reg) {
+#pragma unroll
+                    for (int q = 0; q < kMaxDinReg; ++q) uin[q] = (q < a.din) ? a.u[tt * a.din + q] : 0.0;
+                }
+                const bool aborted = !gather_spikes(a.xch, nw, nb, wpg, a.n, st, a.st0, a.S, bits_s,
+                                                    a.abort_flag, &abort_s, lane, a.G);
+                if (!aborted && lane < nloc && mi == 0) {
+                    double su = 0.0;
+                    if (win_reg) {
+#pragma unroll
+                        for (int q = 0; q < kMaxDinReg; ++q)
+                            if (q < a.din) su = su + win[q] * uin[q];
+                    } else {
+                        for (int q = 0; q < a.din; ++q) su = su + a.W_in[(size_t)i * a.din + q] * a.u[tt * a.din + q];
+                    }
+                    ext_s[tb * nb + lane] = a.gain * su;
+                }
+            }
+            __syncthreads();   // spike bits and ext in LDS (abort_s is only set before this barrier)
+            if (__builtin_amdgcn_readfirstlane(abort_s)) break;
+            if (wave >= 2) {
+                // T_{j+1} = sum_{k=2..j+1} c_k Vb[j+1-k]: wave 2 sums k in [2, km), wave 3
+                // k in [km, j+2); batches of 8, loads from clamped indices first, a term
+                // outside the range gets coefficient 0 (adds an exact zero)
+                if (j >= 1 && j + 1 < K) {
+                    const int km = 2 + (j + 1) / 2;
+                    const int k_lo = wave == 2 ? 2 : km, k_hi = wave == 2 ? km : j + 2;
+                    float h = 0.f;
+                    for (int k = k_lo; k < k_hi; k += 8) {
+                        float cv[8], vv[8];
+#pragma unroll
+                        for (int q = 0; q < 8; ++q) {
+                            cv[q] = c_s[min(k + q, L - 1)];
+                            vv[q] = Vb_s[max(j + 1 - k - q, 0) * nb + lane];
+                        }
+#pragma unroll
+                        for (int q = 0; q < 8; ++q) h = h + (k + q < k_hi ? cv[q] : 0.f) * vv[q];
+                    }
+                    tri_s[((j + 1) & 1) * 2 * nb + (wave - 2) * nb + lane] = h;
+                }
+            }
+            // waves 0 and 1: segment partial sums
+            if (wave < 2)
+                for (int sg = tid; sg < nseg; sg += 2 * 64) {
+                const int b = segr_s[2 * sg], e = segr_s[2 * sg + 1];
+                double su = 0.0;
+                if (!csr_lds) {
+                    su = row_pull_ileave(a.stype, b, e, a.col, a.val, bits_s, a.G, wpg, a.edge_decay, a.edge_tr);
+                } else if (a.stype == ST_SIMPLE) {
+                    // the segment's kSegEdges slots (padded: column code 0, weight 0) as
+                    // b128 loads, then the spike words, then the in-order sum; a silent
+                    // edge's weight is masked to +0.0, which leaves su unchanged (su
+                    // starts at +0.0, so it is never -0.0)
+                    const uint4* ep = reinterpret_cast<const uint4*>(eidx_s + sg * kSegEdges);
+                    const double2* wp = reinterpret_cast<const double2*>(eval_s + sg * kSegEdges);
+                    const uint4 e0 = ep[0], e1 = ep[1];
+                    const double2 w0 = wp[0], w1 = wp[1], w2 = wp[2], w3 = wp[3];
+                    const unsigned pk[8] = {e0.x, e0.y, e0.z, e0.w, e1.x, e1.y, e1.z, e1.w};
+                    const double wv[8] = {w0.x, w0.y, w1.x, w1.y, w2.x, w2.y, w3.x, w3.y};
+                    unsigned bw[8];
+#pragma unroll
+                    for (int q = 0; q < 8; ++q)
+                        bw[q] = *reinterpret_cast<const unsigned*>(reinterpret_cast<const char*>(bits_s) + (pk[q] >> 5));
+#pragma unroll
+                    for (int q = 0; q < 8; ++q) {
+                        const unsigned m = (unsigned)__builtin_amdgcn_sbfe((int)bw[q], pk[q], 1);   // 0 or ~0
+                        const unsigned long long wb = __builtin_bit_cast(unsigned long long, wv[q]) &
+                                                      (((unsigned long long)m << 32) | m);
+                        su = su + __builtin_bit_cast(double, wb);
+                    }
+                } else {   // per-edge synaptic traces in HBM (edge index b + q)
+                    for (int q = 0; q < e - b; ++q) {
+                        const unsigned pk = eidx_s[sg * kSegEdges + q];
+                        const unsigned sbit = (bits_s[pk >> 7] >> (pk & 31)) & 1u;
+                        const int64_t k = b + q;
+                        const double d = a.edge_decay[k];
+                        const double tr = a.edge_tr[k] * d + (1.0 - d) * (double)sbit;
+                        a.edge_tr[k] = tr;
+                        su = su + eval_s[sg * kSegEdges + q] * tr;
+                    }
+                }
+                part_s[sg] = su;
+            }
+            __syncthreads();   // segment partial sums ready
+            if (wave == 0) {
                 if (own) {
-                    const double x = ext_s[tb * nb + lane] + rec_s[sb * nb + lane];
+                    double rec = 0.0;
+                    const int f0 = rowf_s[lane], f1 = rowf_s[lane + 1];
+                    for (int f = f0; f < f1; f += 8) {   // partials in segment order, 8 loads in flight
+                        double pv[8];   // part_s has 8 spare entries past the last segment
+#pragma unroll
+                        for (int q = 0; q < 8; ++q) pv[q] = part_s[f + q];
+#pragma unroll
+                        for (int q = 0; q < 8; ++q)
+                            if (f + q < f1) rec = rec + pv[q];
+                    }
+                    const double x = ext_s[tb * nb + lane] + rec;
                     spk = false;
                     if (tp > p[6]) {
-                        const double rhs = (-(V - p[2]) / p[3]) + x + p[5];
+                        const double rhs = leak + x + p[5];   // (-(V - V_rest) / tau_m) + x + bias
                         V = p[4] * rhs - H;
                         if (V >= p[0]) { V = p[1]; spk = true; tp = 0.0; }
                     } else {
@@ -172,45 +386,16 @@ __global__ void __launch_bounds__(kFastThreads) k_fast(FastArgs a) {
                     tp = tp + a.dt;
                     const float vf = (float)V;
                     Vb_s[j * nb + lane] = vf;
-                    a.ring[(size_t)head * a.n + i] = vf;
+                    vb_last = vf;
+                    a.ring[(size_t)head * a.ldr + (size_t)g * kFastNb + lane] = vf;
                     if (mi == a.micro - 1) {
                         if (a.rec_V) a.rec_V[(size_t)t * a.n + i] = V;
                         if (a.rec_S) a.rec_S[(size_t)t * a.n + i] = spk ? 1 : 0;
                     }
                 }
-            } else if (wave == 1) {
-                double uin[kMaxDinReg];
-                if (mi == 0 && win_reg) {
-#pragma unroll
-                    for (int q = 0; q < kMaxDinReg; ++q) uin[q] = (q < a.din) ? a.u[tt * a.din + q] : 0.0;
-                }
-                const bool aborted = !gather_spikes(a.xch, nw, nb, wpg, a.n, st, a.st0, a.S, bits_s, a.abort_flag,
-                                                    &abort_s, lane);
-                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-                __builtin_amdgcn_wave_barrier();
-                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-                if (!aborted && lane < nloc) {
-                    rec_s[sb * nb + lane] = row_pull(a.stype, csr_lds, rs, re, E0, eidx_s, eval_s, a.col, a.val,
-                                                     bits_s, nb, wpg, a.edge_decay, a.edge_tr);
-                    if (mi == 0) {
-                        double su = 0.0;
-                        if (win_reg) {
-#pragma unroll
-                            for (int q = 0; q < kMaxDinReg; ++q)
-                                if (q < a.din) su = su + win[q] * uin[q];
-                        } else {
-                            for (int q = 0; q < a.din; ++q) su = su + a.W_in[(size_t)i * a.din + q] * a.u[tt * a.din + q];
-                        }
-                        ext_s[tb * nb + lane] = a.gain * su;
-                    }
-                }
-                __syncthreads();
-                if (__builtin_amdgcn_readfirstlane(abort_s)) break;
-            } else {
-                __syncthreads();
-                if (__builtin_amdgcn_readfirstlane(abort_s)) break;
             }
             ++st;
+            if (++head == L) head = 0;
             if (++mi == a.micro) { mi = 0; ++t; }
         }
         if (__builtin_amdgcn_readfirstlane(abort_s)) break;
@@ -223,13 +408,23 @@ __global__ void __launch_bounds__(kFastThreads) k_fast(FastArgs a) {
     }
 }
 
-__global__ void k_ring_f64_to_f32(size_t cnt, const double* src, float* dst) {
+// FP64 ring [L][n] (natural order) -> FP32 ring [L][64 G] (workgroup order; the
+// columns of lanes without a neuron are zero), and back.
+__global__ void k_ring_f64_to_f32(int L, int n, int G, const double* src, float* dst) {
+    const size_t ldr = (size_t)G * kFastNb;
     const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q < cnt) dst[q] = (float)src[q];
+    if (q >= (size_t)L * ldr) return;
+    const size_t s = q / ldr;
+    const int p = (int)(q - s * ldr);
+    const int i = (p / kFastNb) + G * (p % kFastNb);
+    dst[q] = i < n ? (float)src[s * n + i] : 0.f;
 }
-__global__ void k_ring_f32_to_f64(size_t cnt, const float* src, double* dst) {
+__global__ void k_ring_f32_to_f64(int L, int n, int G, const float* src, double* dst) {
     const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q < cnt) dst[q] = (double)src[q];
+    if (q >= (size_t)L * n) return;
+    const size_t s = q / n;
+    const int i = (int)(q - s * n);
+    dst[q] = (double)src[s * (size_t)G * kFastNb + (size_t)(i % G) * kFastNb + i / G];
 }
 
 int fast_supported(spires_dev* g) {
@@ -248,20 +443,21 @@ int fast_supported(spires_dev* g) {
 // Converts the voltage history between the exact (FP64) and fast (FP32) rings
 // when the mode changes.
 int fast_sync_rings(spires_dev* g, bool to_fast) {
-    const size_t cnt = (size_t)g->L * g->n;
+    const int G = (g->n + kFastNb - 1) / kFastNb;
+    const size_t cnt = (size_t)g->L * g->n, cnt32 = (size_t)g->L * G * kFastNb;
     if (cnt == 0) return 0;
     if (to_fast) {
-        if (g->ring32_cap < cnt) {
+        if (g->ring32_cap < cnt32) {
             dfree(g->ring32);
-            int rc = dalloc(&g->ring32, cnt);
+            int rc = dalloc(&g->ring32, cnt32);
             if (rc) return rc;
-            g->ring32_cap = cnt;
+            g->ring32_cap = cnt32;
         }
-        hipLaunchKernelGGL(k_ring_f64_to_f32, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, g->stream, cnt,
-                           g->ring, g->ring32);
+        hipLaunchKernelGGL(k_ring_f64_to_f32, dim3((unsigned)((cnt32 + 255) / 256)), dim3(256), 0, g->stream, g->L,
+                           g->n, G, g->ring, g->ring32);
     } else if (g->ring32) {
-        hipLaunchKernelGGL(k_ring_f32_to_f64, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, g->stream, cnt,
-                           g->ring32, g->ring);
+        hipLaunchKernelGGL(k_ring_f32_to_f64, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, g->stream, g->L,
+                           g->n, G, g->ring32, g->ring);
     }
     SDEV_CK(hipGetLastError());
     return 0;
@@ -296,6 +492,38 @@ int launch_fast(spires_dev* g, const double* u_series, long long t0, long long T
         if (rc) return rc;
         g->ring32_valid = 1;
     }
+    if (!g->fseg) {   // row-pull segments, built once per reservoir (the CSR does not change)
+        if (g->rp_host[g->n] > (int64_t)INT32_MAX) { set_error("fast step kernel: more than 2^31 - 1 synapses"); return SPIRES_DEV_EINVAL; }
+        std::vector<int> off(G + 1, 0), rowf((size_t)G * (nb + 1), 0), rng;
+        int smax = 0;
+        for (int w = 0; w < G; ++w) {
+            int cnt = 0;
+            for (int r = 0; r < nb; ++r) {
+                rowf[(size_t)w * (nb + 1) + r] = cnt;
+                const int64_t row = w + (int64_t)G * r;   // neuron of lane r
+                if (row >= g->n) continue;
+                const int64_t b = g->rp_host[row], e = g->rp_host[row + 1];
+                for (int64_t q = b; q < e; q += kSegEdges) {
+                    rng.push_back((int)q);
+                    rng.push_back((int)std::min<int64_t>(e, q + kSegEdges));
+                    ++cnt;
+                }
+            }
+            rowf[(size_t)w * (nb + 1) + nb] = cnt;
+            off[w + 1] = off[w] + cnt;
+            smax = std::max(smax, cnt);
+        }
+        std::vector<int> tab;
+        tab.reserve(off.size() + rowf.size() + rng.size());
+        tab.insert(tab.end(), off.begin(), off.end());
+        tab.insert(tab.end(), rowf.begin(), rowf.end());
+        tab.insert(tab.end(), rng.begin(), rng.end());
+        int rc = dalloc(&g->fseg, tab.size());
+        if (rc) return rc;
+        SDEV_CK(hipMemcpyAsync(g->fseg, tab.data(), tab.size() * 4, hipMemcpyHostToDevice, g->stream));
+        SDEV_CK(hipStreamSynchronize(g->stream));
+        g->fseg_max = std::max(1, smax);
+    }
     FastArgs a;
     memset(&a, 0, sizeof(a));
     a.n = g->n; a.din = g->din; a.L = g->L; a.nb = nb; a.G = G; a.wpg = wpg; a.micro = g->micro; a.K = K;
@@ -306,17 +534,23 @@ int launch_fast(spires_dev* g, const double* u_series, long long t0, long long T
     a.rp = g->rp; a.col = g->col; a.val = g->val;
     a.edge_decay = g->edge_decay; a.edge_tr = g->edge_tr;
     a.V = g->V; a.S = g->S[g->cur]; a.S_out = g->S[g->cur ^ 1]; a.tprev = g->tprev;
-    a.ring = g->ring32; a.coef = g->coef;
+    a.ring = g->ring32; a.ldr = G * kFastNb; a.coef = g->coef;
     a.xch = g->xch; a.abort_flag = g->abort_flag;
     a.rec_V = rec_V; a.rec_S = rec_S;
-    const int Le = g->L + (g->L & 1);
-    const size_t base = (size_t)Le * 4 + (size_t)2 * K * nb * 4 + 4 * (size_t)nb * 8 + nw * 4 + 64;
-    int64_t me = 0;
-    for (int i0 = 0; i0 < g->n; i0 += nb)
-        me = std::max<int64_t>(me, g->rp_host[std::min(g->n, i0 + nb)] - g->rp_host[i0]);
+    a.seg = g->fseg; a.seg_max = g->fseg_max;
+    // dynamic LDS of k_fast (the layout at the top of the kernel) for ne CSR slots
+    auto lds_bytes = [&](size_t ne) {
+        const size_t f32 = (size_t)((nw + 3) & ~3) + (size_t)((g->L + 3) & ~3) + (size_t)((K * kOStride + 3) & ~3) +
+                           (size_t)K * nb + 4 * (size_t)nb;
+        const size_t f64 = (size_t)((g->fseg_max + 8 + 1) & ~1) + 2 * (size_t)nb + ne;
+        const size_t i32 = ne + 2 * (size_t)g->fseg_max + nb + 1;
+        return f32 * 4 + f64 * 8 + i32 * 4;
+    };
     const size_t cap = 160 * 1024 - 2048;
-    const int ne_lds = (base + (size_t)me * 12 <= cap) ? (int)std::max<int64_t>(me, 1) : 0;
-    const size_t lds = base + (size_t)ne_lds * 12;
+    const int me = kSegEdges * g->fseg_max;   // CSR slots, kSegEdges per segment
+    const int ne_lds = lds_bytes(me) <= cap ? me : 0;
+    const size_t lds = lds_bytes(ne_lds);
+    if (lds > 160 * 1024) { set_error("fast step kernel: history length and segment tables exceed the LDS"); return SPIRES_DEV_EINVAL; }
     a.ne_lds = ne_lds;
     void* args[] = {&a};
     SDEV_CK(hipFuncSetAttribute((const void*)k_fast, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));

@@ -126,6 +126,8 @@ struct spires_dev {
     bool abort_pending = false;       // an asynchronous launch's abort flag is still to be checked
     bool abort_dirty = true;          // abort_flag must be cleared before the next launch
     float* ring32 = nullptr; size_t ring32_cap = 0;   // FP32 history of the fast mode
+    int* fseg = nullptr;        // fast mode's row-pull segments (dev_fast.hip)
+    int fseg_max = 0;           // most segments in one workgroup
     int ring32_valid = 0, ring64_valid = 1;
 };
 
