@@ -5,13 +5,16 @@
 //   H[n0+j] = sum_{m=1}^{min(n0, L-1-j)} c_{m+j} V[n0-m]      (samples before the block)
 //           + sum_{k=1}^{j} c_k V[n0+j-k]                      (in-block triangle)
 // The first part, for all j at once, is the [nb x L] . [L x K] product of the
-// workgroup's FP32 voltage history (ring [L][N] in HBM, slot = time mod L)
+// workgroup's FP32 voltage history (ring in HBM, slot = time mod L)
 // with a Toeplitz coefficient matrix B[s][j] = c_{m(s)+j}, m(s) = (n0-s) mod L,
 // generated on the fly from the coefficients in LDS.  Four waves compute it
 // with v_mfma_f32_16x16x4f32 at each block start (below); then the K steps run like
 // the exact persistent kernel (spike exchange on wave 1) with only the j-term
 // triangle summed step-wise from LDS.  The membrane update, threshold
-// comparison and reset stay FP64 (V = pow(dt,alpha) rhs - H).
+// comparison and reset stay FP64 (V = pow(dt,alpha) rhs - H), with rhs grouped
+// as ((leak + ext) + bias) + rec, so that only the last addition waits for the
+// exchange; the results are deterministic but the additions are not in the
+// exact kernel's order.
 //
 // Neurons are interleaved over the G workgroups: neuron i belongs to workgroup
 // i mod G as its lane i div G.  The hubs of a scale-free graph (the lowest
@@ -26,12 +29,11 @@
 //
 // In-block history: H[n0 + j] = O[j] + c_1 Vb[j-1] + T_j with
 // T_j = sum_{k=2..j} c_k Vb[j-k].  T_{j+1} needs only voltages up to step j - 1,
-// so waves 2 and 3 (one half of the k range each) compute it during step j's
-// row pull, into a double-buffered LDS array; wave 0 adds the three terms at the
-// update.
+// which wave 0 wrote itself, so wave 0 sums it (in a register) while it waits
+// for step j's exchange, and adds the three terms at the next update.
 //
 // Row pull: each row's CSR range is cut into segments of kSegEdges edges; the
-// lanes of waves 0 and 1 sum the segments (each in CSR order, from +0.0) after the
+// lanes of waves 1-3 sum the segments (each in CSR order, from +0.0) after the
 // exchange, and the row's lane adds its segments' partial sums in segment
 // order.  A row of at most kSegEdges edges is one segment, summed exactly as
 // the exact kernel sums it; a hub row of a scale-free graph (hundreds of
@@ -48,8 +50,8 @@ namespace sdev {
 
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 
-constexpr int kFastThreads = 256;     // 4 waves: contraction on all; per step wave 1 exchanges,
-                                      // waves 0-1 pull, waves 2-3 sum the triangle, wave 0 updates
+constexpr int kFastThreads = 256;     // 4 waves: contraction on all; per step wave 0 sums the
+                                      // triangle and updates, wave 1 exchanges, waves 1-3 pull
 constexpr int kFastNb = 64;           // neurons per workgroup (one wave of lanes)
 constexpr int kSegEdges = 8;          // edges per row-pull segment (the pull loads 8 slots)
 constexpr int kOStride = kFastNb + 1; // row stride of O in LDS (padded: conflict-free column writes)
@@ -126,7 +128,7 @@ __global__ void __launch_bounds__(kFastThreads) k_fast(FastArgs a) {
     const int wpg = a.wpg, nw = a.G * wpg;
     // LDS (each part 16-byte aligned): spike bits [nw] (first, at a fixed address),
     // coefficients [L] (FP32), pre-block contraction O [K][nb + 1], in-block voltages
-    // Vb [K][nb], triangle partial sums T [2][2][nb], segment partial sums
+    // Vb [K][nb], segment partial sums
     // [seg_max + 8] (FP64), ext [2][nb] (FP64), CSR values [seg_max][kSegEdges]
     // (FP64), CSR columns [seg_max][kSegEdges] (encode_col_lds), segment ranges
     // [seg_max][2], row segment offsets [nb + 1]
@@ -135,8 +137,7 @@ __global__ void __launch_bounds__(kFastThreads) k_fast(FastArgs a) {
     float* c_s = flds + nwp;
     float* O_s = c_s + Lp;
     float* Vb_s = O_s + ((K * kOStride + 3) & ~3);
-    float* tri_s = Vb_s + K * nb;
-    double* part_s = (double*)(tri_s + 4 * nb);
+    double* part_s = (double*)(Vb_s + K * nb);
     double* ext_s = part_s + ((a.seg_max + 8 + 1) & ~1);
     double* eval_s = ext_s + 2 * nb;
     unsigned* eidx_s = (unsigned*)(eval_s + a.ne_lds);
@@ -175,9 +176,19 @@ __global__ void __launch_bounds__(kFastThreads) k_fast(FastArgs a) {
         if (win_reg)
             for (int j = 0; j < a.din; ++j) win[j] = a.W_in[(size_t)i * a.din + j];
     }
-    const double* p = a.np;
+    // neuron parameters (V_th, V_reset, V_rest, tau_m, pow(dt, alpha), bias, t_ref)
+    // and dt in LDS: read at their point of use, they hold no scalar registers
+    // across the step loop (a kernel-argument array there is spilled and reloaded
+    // as a whole)
+    __shared__ double np_s[8];
+    if (tid < 7) np_s[tid] = a.np[tid];
+    if (tid == 7) np_s[7] = a.dt;
+    const double* p = np_s;
     float vb_last = 0.f;   // wave 0: FP32 voltage of the previous step of the block
+    float tri = 0.f;       // wave 0: T_j of the coming step
     __syncthreads();
+    // wave 0: the row's segment range in part_s (fixed for the launch)
+    const int f0 = own ? rowf_s[lane] : 0, f1 = own ? rowf_s[lane + 1] : 0;
 
     long long st = a.st0;
     long long t = 0;
@@ -260,18 +271,33 @@ __global__ void __launch_bounds__(kFastThreads) k_fast(FastArgs a) {
             const long long tt = a.t0 + t;
             const int tb = (int)(t & 1);
             // wave 0: the history term H and the leak of the update, which need only
-            // the previous steps (T_j was summed during the previous step's pull).
-            // Wave 1: spikes of the previous step and the input drive.
+            // the previous steps, then T_{j+1} for the next step.  Wave 1: spikes of
+            // the previous step and the input drive.
             double H = 0.0, leak = 0.0;
-            if (wave == 0 && own && tp > p[6]) {
-                float h = O_s[j * kOStride + lane];
-                if (j >= 1) h = h + c_s[1] * vb_last;
-                if (j >= 2) {
-                    h = h + tri_s[(j & 1) * 2 * nb + lane];
-                    h = h + tri_s[(j & 1) * 2 * nb + nb + lane];
+            if (wave == 0) {
+                if (own && tp > p[6]) {
+                    float h = O_s[j * kOStride + lane];
+                    if (j >= 1) h = h + c_s[1] * vb_last;
+                    if (j >= 2) h = h + tri;
+                    H = (double)h;
+                    leak = -(V - p[2]) / p[3];
                 }
-                H = (double)h;
-                leak = -(V - p[2]) / p[3];
+                // T_{j+1} = sum_{k=2..j+1} c_k Vb[j+1-k] in batches of 8: loads from
+                // clamped indices first, a term past k = j + 1 gets coefficient 0 (adds
+                // an exact zero)
+                float h = 0.f;
+                if (j + 1 < K)
+                    for (int k = 2; k <= j + 1; k += 8) {
+                        float cv[8], vv[8];
+#pragma unroll
+                        for (int q = 0; q < 8; ++q) {
+                            cv[q] = c_s[min(k + q, L - 1)];
+                            vv[q] = Vb_s[max(j + 1 - k - q, 0) * nb + lane];
+                        }
+#pragma unroll
+                        for (int q = 0; q < 8; ++q) h = h + (k + q <= j + 1 ? cv[q] : 0.f) * vv[q];
+                    }
+                tri = h;
             }
             if (wave == 1) {
                 double uin[kMaxDinReg];
@@ -295,30 +321,13 @@ __global__ void __launch_bounds__(kFastThreads) k_fast(FastArgs a) {
             }
             __syncthreads();   // spike bits and ext in LDS (abort_s is only set before this barrier)
             if (__builtin_amdgcn_readfirstlane(abort_s)) break;
-            if (wave >= 2) {
-                // T_{j+1} = sum_{k=2..j+1} c_k Vb[j+1-k]: wave 2 sums k in [2, km), wave 3
-                // k in [km, j+2); batches of 8, loads from clamped indices first, a term
-                // outside the range gets coefficient 0 (adds an exact zero)
-                if (j >= 1 && j + 1 < K) {
-                    const int km = 2 + (j + 1) / 2;
-                    const int k_lo = wave == 2 ? 2 : km, k_hi = wave == 2 ? km : j + 2;
-                    float h = 0.f;
-                    for (int k = k_lo; k < k_hi; k += 8) {
-                        float cv[8], vv[8];
-#pragma unroll
-                        for (int q = 0; q < 8; ++q) {
-                            cv[q] = c_s[min(k + q, L - 1)];
-                            vv[q] = Vb_s[max(j + 1 - k - q, 0) * nb + lane];
-                        }
-#pragma unroll
-                        for (int q = 0; q < 8; ++q) h = h + (k + q < k_hi ? cv[q] : 0.f) * vv[q];
-                    }
-                    tri_s[((j + 1) & 1) * 2 * nb + (wave - 2) * nb + lane] = h;
-                }
-            }
-            // waves 0 and 1: segment partial sums
-            if (wave < 2)
-                for (int sg = tid; sg < nseg; sg += 2 * 64) {
+            // wave 0, while the others pull: the update's terms that do not depend on
+            // the recurrent input, a0 = (leak + ext) + bias
+            double a0 = 0.0;
+            if (wave == 0 && own && tp > p[6]) a0 = (leak + ext_s[tb * nb + lane]) + p[5];
+            // waves 1-3: segment partial sums
+            if (wave >= 1)
+                for (int sg = tid - 64; sg < nseg; sg += 3 * 64) {
                 const int b = segr_s[2 * sg], e = segr_s[2 * sg + 1];
                 double su = 0.0;
                 if (!csr_lds) {
@@ -361,21 +370,23 @@ __global__ void __launch_bounds__(kFastThreads) k_fast(FastArgs a) {
             __syncthreads();   // segment partial sums ready
             if (wave == 0) {
                 if (own) {
+                    // recurrent input: the row's segment partial sums, 8 at a time added
+                    // as a tree (depth 3; slots past the row hold +0.0 after masking)
                     double rec = 0.0;
-                    const int f0 = rowf_s[lane], f1 = rowf_s[lane + 1];
-                    for (int f = f0; f < f1; f += 8) {   // partials in segment order, 8 loads in flight
+                    for (int f = f0; f < f1; f += 8) {
                         double pv[8];   // part_s has 8 spare entries past the last segment
 #pragma unroll
                         for (int q = 0; q < 8; ++q) pv[q] = part_s[f + q];
 #pragma unroll
-                        for (int q = 0; q < 8; ++q)
-                            if (f + q < f1) rec = rec + pv[q];
+                        for (int q = 0; q < 8; ++q) pv[q] = f + q < f1 ? pv[q] : 0.0;
+                        const double s4a = (pv[0] + pv[1]) + (pv[2] + pv[3]);
+                        const double s4b = (pv[4] + pv[5]) + (pv[6] + pv[7]);
+                        rec = rec + (s4a + s4b);
                     }
-                    const double x = ext_s[tb * nb + lane] + rec;
                     spk = false;
                     if (tp > p[6]) {
-                        const double rhs = leak + x + p[5];   // (-(V - V_rest) / tau_m) + x + bias
-                        V = p[4] * rhs - H;
+                        // V = alpha_term * ((leak + ext + bias) + rec) - H
+                        V = p[4] * (a0 + rec) - H;
                         if (V >= p[0]) { V = p[1]; spk = true; tp = 0.0; }
                     } else {
                         V = p[1];
@@ -383,7 +394,7 @@ __global__ void __launch_bounds__(kFastThreads) k_fast(FastArgs a) {
                 }
                 publish_spikes(a.xch, nw, g, wpg, st, lane, __ballot(own && spk));
                 if (own) {
-                    tp = tp + a.dt;
+                    tp = tp + p[7];
                     const float vf = (float)V;
                     Vb_s[j * nb + lane] = vf;
                     vb_last = vf;
@@ -541,7 +552,7 @@ int launch_fast(spires_dev* g, const double* u_series, long long t0, long long T
     // dynamic LDS of k_fast (the layout at the top of the kernel) for ne CSR slots
     auto lds_bytes = [&](size_t ne) {
         const size_t f32 = (size_t)((nw + 3) & ~3) + (size_t)((g->L + 3) & ~3) + (size_t)((K * kOStride + 3) & ~3) +
-                           (size_t)K * nb + 4 * (size_t)nb;
+                           (size_t)K * nb;
         const size_t f64 = (size_t)((g->fseg_max + 8 + 1) & ~1) + 2 * (size_t)nb + ne;
         const size_t i32 = ne + 2 * (size_t)g->fseg_max + nb + 1;
         return f32 * 4 + f64 * 8 + i32 * 4;
