@@ -196,6 +196,16 @@ int  spires_dev_coarse_grain(int n, const int64_t *row_ptr, const int32_t *col_i
  * them); *lambda gets the unscaled estimate. */
 int  spires_dev_random_er(int n, double p, double ei, unsigned long long seed, double rho, int64_t **row_ptr,
                           int32_t **col_idx, double **values, size_t *nnz, double *lambda);
+/* spires_dev_random_er for any of the three topologies (type: 0 random, 1
+ * small world, 2 scale free; reservoir.c:384-536), with the reference's sizes
+ * from conn (ring degree K, edges per new node m) and its construction steps
+ * in law, from SplitMix64 draws: the rewired ring of the small world, the
+ * preferential attachment (deg + 1) with the seed clique and the uniform
+ * fallback, then a fair-coin orientation, for the scale-free graph.  Then the
+ * same power iteration and rescale.  n <= 19968. */
+int  spires_dev_build_topology(int type, int n, double conn, double ei, unsigned long long seed, double rho,
+                               int64_t **row_ptr, int32_t **col_idx, double **values, size_t *nnz,
+                               double *lambda);
 /* The reference power iteration (sparse.c:93-127) on the device for a host
  * CSR matrix; *lambda is bit-identical to the CPU loop's, *iters its count. */
 int  spires_dev_spectral_radius(int n, const int64_t *row_ptr, const int32_t *col_idx,

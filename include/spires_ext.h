@@ -56,12 +56,26 @@ double spires_ext_spectral_radius(size_t n, const int64_t *row_ptr, const int32_
 spires_status spires_ext_random_er(size_t n, double p, double ei, uint64_t seed, double rho,
                                    size_t *nnz, int64_t **row_ptr, int32_t **col_idx,
                                    double **values, double *lambda);
-/* spires_reservoir_create (spires.h) for SPIRES_CONN_RANDOM with W built by
- * spires_ext_random_er(N, connectivity, ei_ratio, seed, spectral_radius) and
- * W_in / W_out from SplitMix64 streams of the same seed (generate_weight's
- * distribution; ei_ratio for W_in, 0.5 for W_out).  Synapse time constants
- * still come from rand(), as in spires_reservoir_create.  Other topologies
- * return SPIRES_ERR_INVALID_ARG. */
+/* spires_ext_random_er for any connectivity type (replaces the small-world
+ * and scale-free branches of init_weights, reservoir.c:399-536, and the
+ * rescale): the reference's sizes (ring degree K = round(conn (n-1)), even,
+ * K/2 forward neighbours; m = round(conn (n-1)) edges per new node over a
+ * clique of max(m, 2)) and its steps in law -- each ring edge rewired with
+ * probability 0.1 to a uniform target that is neither i nor an existing edge;
+ * preferential attachment proportional to deg + 1 with the uniform fallback
+ * after 50 m attempts, each undirected edge oriented by a fair coin -- from
+ * SplitMix64 draws instead of rand().  Then the reference power iteration and
+ * the rescale to rho on the device.  n <= 19968; outputs malloc'd. */
+spires_status spires_ext_build_topology(spires_connectivity_type type, size_t n, double connectivity,
+                                        double ei, uint64_t seed, double rho, size_t *nnz,
+                                        int64_t **row_ptr, int32_t **col_idx, double **values,
+                                        double *lambda);
+/* spires_reservoir_create (spires.h) with W built by
+ * spires_ext_build_topology(connectivity_type, N, connectivity, ei_ratio,
+ * seed, spectral_radius) and W_in / W_out from SplitMix64 streams of the same
+ * seed (generate_weight's distribution; ei_ratio for W_in, 0.5 for W_out).
+ * Synapse time constants still come from rand(), as in
+ * spires_reservoir_create. */
 spires_status spires_ext_create_device(const spires_reservoir_config *cfg, uint64_t seed,
                                        spires_reservoir **out);
 /* spires_ext_spectral_radius on the device (n <= 19968): the same operations

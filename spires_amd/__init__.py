@@ -144,6 +144,9 @@ SIGNATURES = {
     "spires_ext_random_er": (C.c_int, [_SZ, C.c_double, C.c_double, C.c_uint64, C.c_double, C.POINTER(_SZ),
                                        C.POINTER(_I64),
                                        C.POINTER(_I32), C.POINTER(_D), C.POINTER(C.c_double)]),
+    "spires_ext_build_topology": (C.c_int, [C.c_int, _SZ, C.c_double, C.c_double, C.c_uint64, C.c_double,
+                                            C.POINTER(_SZ), C.POINTER(_I64), C.POINTER(_I32), C.POINTER(_D),
+                                            C.POINTER(C.c_double)]),
     "spires_ext_spectral_radius_device": (C.c_int, [_SZ, _I64, _I32, _D, C.POINTER(C.c_double),
                                                     C.POINTER(C.c_int)]),
     "spires_ext_spectral_radius_device_noalloc": (C.c_int, [_SZ, _P, _P, _P, _P, _P, _P]),
@@ -260,14 +263,7 @@ def spectral_radius_device(rp, ci, val):
     return lam.value, it.value
 
 
-def random_er(n, p, seed, rho, ei=0.8):
-    """Erdos-Renyi recurrent weights built on the GPU: (row_ptr, col_idx,
-    values, lambda) with values scaled to spectral radius rho (lambda is the
-    power-iteration estimate before scaling)."""
-    nnz, lam = _SZ(), C.c_double()
-    rp, ci, val = _I64(), _I32(), _D()
-    _check(lib().spires_ext_random_er(n, float(p), float(ei), C.c_uint64(seed), float(rho), C.byref(nnz), C.byref(rp),
-                                      C.byref(ci), C.byref(val), C.byref(lam)))
+def _take_csr(n, nnz, rp, ci, val, lam):
     k = nnz.value
     try:
         out = (np.ctypeslib.as_array(rp, shape=(n + 1,)).copy(),
@@ -278,6 +274,28 @@ def random_er(n, p, seed, rho, ei=0.8):
         for q in (rp, ci, val):
             _libc.free(C.cast(q, _P))
     return out
+
+
+def random_er(n, p, seed, rho, ei=0.8):
+    """Erdos-Renyi recurrent weights built on the GPU: (row_ptr, col_idx,
+    values, lambda) with values scaled to spectral radius rho (lambda is the
+    power-iteration estimate before scaling)."""
+    nnz, lam = _SZ(), C.c_double()
+    rp, ci, val = _I64(), _I32(), _D()
+    _check(lib().spires_ext_random_er(n, float(p), float(ei), C.c_uint64(seed), float(rho), C.byref(nnz), C.byref(rp),
+                                      C.byref(ci), C.byref(val), C.byref(lam)))
+    return _take_csr(n, nnz, rp, ci, val, lam)
+
+
+def build_topology(topology, n, conn, seed, rho, ei=0.8):
+    """Recurrent weights of any topology ("random", "small_world",
+    "scale_free") built on the GPU (spires_ext_build_topology): (row_ptr,
+    col_idx, values, lambda), values scaled to spectral radius rho."""
+    nnz, lam = _SZ(), C.c_double()
+    rp, ci, val = _I64(), _I32(), _D()
+    _check(lib().spires_ext_build_topology(CONN[topology], n, float(conn), float(ei), C.c_uint64(seed), float(rho),
+                                           C.byref(nnz), C.byref(rp), C.byref(ci), C.byref(val), C.byref(lam)))
+    return _take_csr(n, nnz, rp, ci, val, lam)
 
 
 def build_weights(cfg_tuple):
@@ -439,7 +457,7 @@ class Reservoir:
 
     def __init__(self, cfg_tuple, csr=None, W_in=None, W_out=None, device_seed=None):
         """csr=(row_ptr, col_idx, values) with W_in/W_out: given weights;
-        device_seed: random topology built on the GPU from that seed
+        device_seed: recurrent weights (any topology) built on the GPU from that seed
         (spires_ext_create_device); neither: spires_reservoir_create."""
         self._keep = cfg_tuple
         cfg = cfg_tuple[0]

@@ -776,6 +776,19 @@ spires_status spires_ext_random_er(size_t n, double p, double ei, uint64_t seed,
                                            nnz, lambda));
 }
 
+// init_weights' topology branches (reservoir.c:384-536) and rescale_weights
+// (reservoir.c:548-557) on the device, for any connectivity type.
+spires_status spires_ext_build_topology(spires_connectivity_type type, size_t n, double connectivity, double ei,
+                                        uint64_t seed, double rho, size_t* nnz, int64_t** row_ptr, int32_t** col_idx,
+                                        double** values, double* lambda) {
+    if (n == 0 || n > (size_t)INT32_MAX || !nnz || !row_ptr || !col_idx || !values) return SPIRES_ERR_INVALID_ARG;
+    if (type != SPIRES_CONN_RANDOM && type != SPIRES_CONN_SMALL_WORLD && type != SPIRES_CONN_SCALE_FREE)
+        return SPIRES_ERR_INVALID_ARG;
+    const int t = type == SPIRES_CONN_RANDOM ? 0 : type == SPIRES_CONN_SMALL_WORLD ? 1 : 2;
+    return dev_status(spires_dev_build_topology(t, (int)n, connectivity, ei, (unsigned long long)seed, rho, row_ptr,
+                                                col_idx, values, nnz, lambda));
+}
+
 // init_reservoir (reservoir.c:68-78) with the recurrent matrix built and
 // rescaled on the device.  W_in and W_out keep generate_weight's distribution
 // (magnitude U(0,1], sign by ei_ratio; 0.5 for W_out, reservoir.c:559-571),
@@ -784,17 +797,13 @@ spires_status spires_ext_create_device(const spires_reservoir_config* cfg, uint6
     if (!cfg || !out_r) return SPIRES_ERR_INVALID_ARG;
     spires_status s = validate_cfg(cfg);
     if (s != SPIRES_OK) return s;
-    if (cfg->connectivity_type != SPIRES_CONN_RANDOM) {
-        set_error("device construction builds the random (Erdos-Renyi) topology only");
-        return SPIRES_ERR_INVALID_ARG;
-    }
     const size_t n = cfg->num_neurons;
     size_t nnz = 0;
     int64_t* rp = nullptr;
     int32_t* ci = nullptr;
     double* val = nullptr;
-    s = spires_ext_random_er(n, cfg->connectivity, cfg->ei_ratio, seed, cfg->spectral_radius, &nnz, &rp, &ci, &val,
-                             nullptr);
+    s = spires_ext_build_topology(cfg->connectivity_type, n, cfg->connectivity, cfg->ei_ratio, seed,
+                                  cfg->spectral_radius, &nnz, &rp, &ci, &val, nullptr);
     if (s != SPIRES_OK) return s;
     std::vector<double> W_in(n * cfg->num_inputs), W_out(n * cfg->num_outputs);
     uint64_t z = seed ^ 0xD1B54A32D192ED03ULL;

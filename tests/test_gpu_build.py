@@ -23,6 +23,7 @@ import spires_amd as sa
 from spires_amd import workloads
 from oracle import pyoracle as po
 from tests import er_restate as er
+from tests import sw_ba_restate as sb
 
 pytestmark = pytest.mark.gpu
 
@@ -118,9 +119,61 @@ def test_create_device_reservoir():
     assert np.isfinite(e_dev) and 0.5 * e_ref < e_dev < 2.0 * e_ref
 
 
-def test_create_device_rejects_other_topologies():
-    with pytest.raises(RuntimeError):
-        sa.Reservoir(sa.make_config(256, conn=0.1, conn_type="small_world"), device_seed=1)
+@pytest.mark.parametrize("topo,n,conn,ei,seed", [
+    ("small_world", 600, 20 / 599, 0.8, 31), ("small_world", 97, 0.5, 0.5, 32), ("small_world", 3, 0.5, 0.8, 33),
+    ("small_world", 2, 0.5, 0.8, 34), ("scale_free", 500, 4 / 499, 0.8, 35), ("scale_free", 300, 0.05, 0.3, 36),
+    ("scale_free", 2, 0.5, 0.8, 37), ("scale_free", 1, 0.5, 0.8, 38), ("random", 400, 0.05, 0.8, 39)])
+def test_build_topology_bit_exact(topo, n, conn, ei, seed):
+    """Device small-world and scale-free construction against the restatement
+    in tests/sw_ba_restate.py (whose law tests/test_sw_ba_restate.py checks
+    against the reference's construction): CSR, lambda and values bit for bit,
+    the degenerate sizes (no edges below 3 or 2 neurons) included."""
+    print(f"case: {topo} n={n} conn={conn:.4f}", flush=True)
+    rp, ci, val, lam = sa.build_topology(topo, n, conn, seed=seed, rho=0.9, ei=ei)
+    rp_e, ci_e, val_e, lam_e = sb.build(topo, n, conn, ei, seed, 0.9)
+    print(f"  nnz={len(ci)} lambda={lam!r}", flush=True)
+    assert np.array_equal(rp, rp_e) and np.array_equal(ci, ci_e)
+    assert bits(lam) == bits(lam_e)
+    assert np.array_equal(val.view(np.uint64), val_e.view(np.uint64))
+
+
+@pytest.mark.parametrize("topo,k", [("small_world", 20), ("scale_free", 8)])
+def test_create_device_sw_ba_reservoir(topo, k):
+    """FLIF-GL reservoirs whose small-world or scale-free W is built on the
+    device: the CSR equals the restatement, the run is bit-exact against the
+    oracle on those weights."""
+    N, L, T = 640, 64, 400
+    params = [1.0, 0.0, 0.0, 20.0, 0.8, float(L), 0.05, 2.0]
+    kw = dict(din=1, dout=1, rho=0.95, gain=2.0, conn=k / (N - 1), conn_type=topo, neuron="flif_gl", params=params)
+    r = sa.Reservoir(sa.make_config(N, **kw), device_seed=41)
+    rp, ci, val = r.csr()
+    rp_e, ci_e, val_e, _ = sb.build(topo, N, k / (N - 1), 0.8, 41, 0.95)
+    assert np.array_equal(rp, rp_e) and np.array_equal(ci, ci_e) and np.array_equal(val, val_e)
+    u = np.random.default_rng(3).uniform(0, 0.5, (T, 1))
+    o = po.OracleReservoir(po.make_config(N, **kw), csr=(rp, ci, val), W_in=r.W_in())
+    X, R, _ = r.run_record(u)
+    Xo, Ro, _ = o.run_record(u, T)
+    print(f"{topo}: nnz {len(ci)}, spikes {int(R.sum())}", flush=True)
+    assert Ro.sum() > 0
+    assert np.array_equal(R, Ro)
+    assert np.array_equal(X.view(np.uint64), Xo.view(np.uint64))
+    r.close()
+
+
+@pytest.mark.parametrize("topo,k", [("small_world", 20), ("scale_free", 8)])
+def test_topology_construction_time(topo, k):
+    """Device vs host (the reference's rand() order) construction at N=16384."""
+    n = 16384
+    t0 = time.perf_counter()
+    rp, ci, val, lam = sa.build_topology(topo, n, k / (n - 1), seed=5, rho=0.9)
+    t_dev = time.perf_counter() - t0
+    sa.srand(5)
+    t0 = time.perf_counter()
+    rp_h, ci_h, _, _, _ = sa.build_weights(sa.make_config(n, conn=k / (n - 1), conn_type=topo, rho=0.9))
+    t_host = time.perf_counter() - t0
+    print(f"N={n} {topo}: device {t_dev:.3f} s (nnz {len(ci)}), host {t_host:.3f} s (nnz {len(ci_h)})", flush=True)
+    assert len(ci) == len(ci_h)
+    assert np.isfinite(lam) and lam > 0
 
 
 def test_construction_time():
