@@ -1607,17 +1607,19 @@ extern "C" int spires_dev_timeblock_create(spires_dev* const* gs, int count, lon
     if (e == hipSuccess && lds32) e = hipFuncSetAttribute((const void*)k_persist_tp_batch<32>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds32);
     if (e == hipSuccess && lds16) e = hipFuncSetAttribute((const void*)k_persist_tp_batch<16>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds16);
     if (e != hipSuccess) { set_error(hipGetErrorString(e)); return fail(SPIRES_DEV_EINTERNAL); }
-    // capture one block: each launch, then the clock advance of its reservoirs
+    // capture one block: each launch (cooperative, so that the runtime checks that
+    // all its workgroups can be resident at once, as the spike exchange needs),
+    // then the clock advance of its reservoirs
     e = hipStreamBeginCapture(tb->stream, hipStreamCaptureModeThreadLocal);
     for (size_t q = 0; q < launches.size() && e == hipSuccess; ++q) {
         const int G = (int)((q + 1 < launches.size() ? map_off[q + 1] : map.size()) - map_off[q]);
         const int cnt = (int)launches[q].size();
-        if (nbs[q] == 32)
-            hipLaunchKernelGGL(k_persist_tp_batch<32>, dim3(G), dim3(kTpThreads), (unsigned)lds[q], tb->stream,
-                               (const PersistArgs*)tb->table, (const int2*)(tb->map + map_off[q]));
-        else
-            hipLaunchKernelGGL(k_persist_tp_batch<16>, dim3(G), dim3(kTpThreads), (unsigned)lds[q], tb->stream,
-                               (const PersistArgs*)tb->table, (const int2*)(tb->map + map_off[q]));
+        const PersistArgs* tab = tb->table;
+        const int2* mp = tb->map + map_off[q];
+        void* kargs[] = {(void*)&tab, (void*)&mp};
+        const void* fn = nbs[q] == 32 ? (const void*)k_persist_tp_batch<32> : (const void*)k_persist_tp_batch<16>;
+        e = hipLaunchCooperativeKernel(fn, dim3(G), dim3(kTpThreads), kargs, (unsigned)lds[q], tb->stream);
+        if (e != hipSuccess) break;
         hipLaunchKernelGGL(k_timeblock_advance, dim3(1), dim3(64 * ((cnt + 63) / 64)), 0, tb->stream,
                            (const TbItem*)(tb->items + item_off[q]), cnt);
         e = hipGetLastError();
