@@ -284,6 +284,10 @@ def run_batched(args):
     singles = [i for i in range(len(res)) if plans[i]["batch_nb"] == 0]
     total_n = sum(sp["n"] for sp in specs)
     flags = sa.REC_STATE | sa.REC_APPEND
+    # the graph's reservoirs and flags: the product-wave launches, or (one launch)
+    # every reservoir as 64-neuron one-wave slices of a single launch
+    gsel, gflags, gsingles = (list(range(len(res))), sa.REC_STATE | sa.TB_ONE_LAUNCH, []) if args.one_launch \
+        else (tp, sa.REC_STATE, singles)
 
     def eager_block(t, K):
         """The same launches without the graph (run_resident_batch per launch)."""
@@ -305,20 +309,20 @@ def run_batched(args):
         for g in groups:
             sa.run_resident_batch([res[i] for i in g], t, K, flags)
 
-    def single_block(t, K):
-        for i in singles:
+    def single_block(t, K, which=None):
+        for i in (gsingles if which is None else which):
             res[i].run_resident(t, K, flags)
 
     # untimed warm-up until every history is full
     t = 1024
-    tb = sa.TimeBlock([res[i] for i in tp], t, sa.REC_STATE)
+    tb = sa.TimeBlock([res[i] for i in gsel], t, gflags)
     tb.run(1)
     tb.close()
     single_block(0, t)
     # K autotune: one graph block per candidate (each advances every reservoir)
     ktimes = {}
     for K in (64, 256, 1024):
-        tb = sa.TimeBlock([res[i] for i in tp], K, sa.REC_STATE)
+        tb = sa.TimeBlock([res[i] for i in gsel], K, gflags)
         t0 = time.perf_counter()
         tb.run(1)
         single_block(t, K)
@@ -330,12 +334,12 @@ def run_batched(args):
     t0 = time.perf_counter()
     for _ in range(2):
         eager_block(t, K)
-        single_block(t, K)
+        single_block(t, K, singles)
         t += K
     eager_us = (time.perf_counter() - t0) / (2 * K) * 1e6
     # timed: graph replays of whole blocks
-    tb = sa.TimeBlock([res[i] for i in tp], K, sa.REC_STATE)
-    launches = tb.launches + len(singles)
+    tb = sa.TimeBlock([res[i] for i in gsel], K, gflags)
+    launches = tb.launches + len(gsingles)
     blocks = (T - t) // K
     t0 = time.perf_counter()
     for _ in range(blocks):
@@ -356,14 +360,18 @@ def run_batched(args):
         "config": {"workload": "64 FLIF-GL reservoirs, N log-U[256,8192] (total %d neurons), ER p=0.05 / BA m=5, "
                                "alpha U[0.5,1], T_mem in {128,256,512,1024}, rho U[0.7,1.2], N(0,1) weights; "
                                "13-channel AR(1) class sequences (512 sequences, %d frames)" % (total_n, T),
-                   "block_k": K, "graph": "HIP graph per time block (%d product-wave reservoirs)" % len(tp)},
+                   "block_k": K,
+                   "graph": ("HIP graph per time block: one launch of 64-neuron slices (all %d reservoirs)" % len(gsel))
+                   if args.one_launch else "HIP graph per time block (%d product-wave reservoirs)" % len(tp)},
         "k_autotune_us_per_step": ktimes,
         "eager_us_per_step": eager_us,
         "launches_per_block": launches,
-        "single_launches": [dict(n=specs[i]["n"], T_mem=specs[i]["T_mem"], kernel=plans[i]["kernel"]) for i in singles],
+        "single_launches": [dict(n=specs[i]["n"], T_mem=specs[i]["T_mem"], kernel=plans[i]["kernel"]) for i in gsingles],
         "ms_per_step": wall / steps * 1e3,
         "roofline": {"bound": "hbm", "achieved": alg, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": alg / HBM_PEAK_GBS,
-                     "traffic": None, "kernel": "k_persist_tp_batch (graph) + k_persist (all launches of a block)"},
+                     "traffic": None,
+                     "kernel": "k_mux_gl (one launch per block)" if args.one_launch
+                     else "k_persist_tp_batch (graph) + k_persist (all launches of a block)"},
     }
     print(json.dumps(line), flush=True)
 
@@ -765,6 +773,8 @@ def main():
                     help="time one batched ridge fit of 16 N=2048 search trials at the 6000-step rung")
     ap.add_argument("--batched", action="store_true",
                     help="run the batched workload (64 uneven reservoirs) instead of the step benchmark")
+    ap.add_argument("--one-launch", action="store_true",
+                    help="with --batched: every reservoir in one launch of 64-neuron slices per time block")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the library runs one reservoir per device and has no "

@@ -104,6 +104,9 @@ int  spires_dev_batch_workgroups(spires_dev *const *gs, int count);
  * msteps + k; with REC_STATE / REC_RASTER, records are written at row
  * msteps + k (reserve them beforehand; reallocating inputs or records
  * invalidates the time block). */
+#define SPIRES_DEV_TB_ONE_LAUNCH 16u   /* time block: every reservoir in one launch of 64-neuron
+                                          one-wave slices (k_mux_gl) instead of packed
+                                          product-wave launches */
 typedef struct spires_dev_timeblock spires_dev_timeblock;
 int  spires_dev_timeblock_create(spires_dev *const *gs, int count, long long K, unsigned flags,
                                  spires_dev_timeblock **out);

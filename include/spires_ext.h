@@ -159,6 +159,10 @@ int           spires_ext_batch_workgroups(spires_reservoir *const *rs, int count
  * stopped.  Inputs are read, and records (flags SPIRES_EXT_REC_STATE /
  * _RASTER) written, at row = macro step since reset; reserve the records
  * first.  No counterpart in the reference. */
+#define SPIRES_EXT_TB_ONE_LAUNCH 16u   /* all reservoirs in one persistent launch: one wave per
+                                          64 neurons, the history and CSR streamed from HBM;
+                                          reservoirs of at most 16384 neurons whose slices are
+                                          resident together */
 typedef struct spires_timeblock spires_timeblock;
 spires_status spires_ext_timeblock_create(spires_reservoir *const *rs, int count, size_t K, unsigned flags,
                                           spires_timeblock **out);

@@ -26,6 +26,7 @@ BACKEND = {"sparse": 0, "dense": 1}
 STATUS = {0: "SPIRES_OK", 1: "SPIRES_ERR_INVALID_ARG", 2: "SPIRES_ERR_ALLOC", 3: "SPIRES_ERR_INTERNAL"}
 
 REC_STATE, REC_RASTER, REC_OUTPUT, REC_APPEND = 1, 2, 4, 8
+TB_ONE_LAUNCH = 16   # TimeBlock: every reservoir in one launch of 64-neuron one-wave slices
 MODE_EXACT, MODE_FAST = 0, 1
 PATH_AUTO, PATH_PER_STEP, PATH_PERSISTENT, PATH_PERSISTENT_HBM, PATH_PERSISTENT_LDS = 0, 1, 2, 3, 4
 

@@ -1465,6 +1465,281 @@ __global__ void k_timeblock_advance(const TbItem* __restrict__ items, int count)
     *it.abort_flag = 0u;
 }
 
+// ---------------------------------------------------------------------------
+// All reservoirs of a time block in one persistent launch.  Workgroup b is one
+// wave that owns the 64-neuron slice wg_map[b].y of reservoir wg_map[b].x for
+// the whole block: lane = neuron.  Per micro-step the wave sums the GL history
+// from the ring in HBM (the single-wave chain of k_persist, lags in ascending
+// order), gathers the reservoir's spike words of the previous step, pulls its
+// CSR rows from HBM (CSR order), forms the input drive and updates the neurons
+// (flif_gl.c:50-100), then publishes its two spike words.  Reservoirs share
+// nothing but the launch: a reservoir with a short history runs ahead of one
+// with a long history, and the launch lasts as long as the slowest reservoir,
+// not the sum over reservoirs.  Every slice of every reservoir is resident at
+// once (cooperative launch; the host checks the occupancy).
+
+// ---- one-launch kernel helpers: global-memory pointers ----------------------
+// The reservoir arguments come from a table in memory, so the compiler cannot
+// infer that their pointers address global memory and would use FLAT accesses
+// (a 64-bit address per lane and per load, both wait counters).  The kernel
+// converts them to address-space-1 pointers once and these helpers take them
+// as such; each does what its counterpart in dev_exchange.hpp / gl_seg_hbm
+// does, in the same order.
+#define SPIRES_GLOBAL __attribute__((address_space(1)))
+template <typename T>
+using gptr = SPIRES_GLOBAL T*;
+
+template <typename T>
+__device__ __forceinline__ gptr<T> to_global(T* p) {
+    return (gptr<T>)p;
+}
+
+// gl_seg_hbm with each slot's base a wave-uniform pointer and the lane's
+// neuron a 32-bit byte offset: the loads take a scalar base and one offset
+// register instead of a 64-bit address per lane and load.
+template <int NBLK>
+__device__ __forceinline__ double gl_seg_mux(double h, gptr<const double> ring, unsigned off,
+                                             gptr<const double> c, int kb, int sb, int cnt, size_t stride) {
+    constexpr int B = 16;
+    const int nblk = cnt / B;
+    double v[NBLK][B];
+    auto at = [&](int slot) {
+        const int sl = __builtin_amdgcn_readfirstlane(slot);   // uniform: a scalar base per slot
+        gptr<const char> base = (gptr<const char>)(ring + (size_t)sl * stride);
+        return *(gptr<const double>)(base + off);
+    };
+    auto load = [&](double* buf, int blk) {
+#pragma unroll
+        for (int j = 0; j < B; ++j) buf[j] = at(sb - blk * B - j);
+    };
+#pragma unroll
+    for (int u = 0; u < NBLK; ++u)
+        if (u < nblk) load(v[u], u);
+    for (int b0 = 0; b0 < nblk; b0 += NBLK) {
+#pragma unroll
+        for (int u = 0; u < NBLK; ++u) {
+            const int blk = b0 + u;
+            if (blk < nblk) {
+#pragma unroll
+                for (int j = 0; j < B; ++j) h = h + c[kb + blk * B + j] * v[u][j];
+                if (blk + NBLK < nblk) load(v[u], blk + NBLK);
+            }
+        }
+    }
+    const int q = nblk * B, rem = cnt - q;
+    for (int j = 0; j < rem; ++j) h = h + c[kb + q + j] * at(sb - q - j);
+    return h;
+}
+
+// gather_spikes for 64-neuron slices (two words each): spike bits of micro-step
+// st-1 of one reservoir into bits_s[nw]; at the launch's first step from S.
+__device__ __forceinline__ bool mux_gather(gptr<unsigned long long> xch, int nw, int n, long long st, long long st0,
+                                           gptr<const uint8_t> S, unsigned* bits_s, gptr<unsigned> abort_flag,
+                                           int* abort_s, int lane) {
+    if (st == st0) {
+        for (int w = lane; w < nw; w += 64) {
+            unsigned b = 0;
+            for (int j = 0; j < 32; ++j) {
+                const int nj = w * 32 + j;   // slice w/2, lanes 32 (w%2) + j
+                if (nj < n && S[nj]) b |= 1u << j;
+            }
+            bits_s[w] = b;
+        }
+        return true;
+    }
+    gptr<const unsigned long long> slot = xch + (size_t)((st - 1) & 1) * nw;
+    const unsigned tag = (unsigned)st;
+    unsigned long long word[kMaxWordsPerLane];
+    bool ready[kMaxWordsPerLane];
+#pragma unroll
+    for (int q = 0; q < kMaxWordsPerLane; ++q) { ready[q] = lane + 64 * q >= nw; word[q] = 0; }
+    long long spins = 0;
+    bool ok = true;
+    while (true) {
+#pragma unroll
+        for (int q = 0; q < kMaxWordsPerLane; ++q)
+            if (!ready[q]) word[q] = __hip_atomic_load(&slot[lane + 64 * q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        bool all_here = true;
+#pragma unroll
+        for (int q = 0; q < kMaxWordsPerLane; ++q) {
+            if (!ready[q]) ready[q] = (unsigned)(word[q] >> 32) == tag;
+            all_here = all_here && ready[q];
+        }
+        if (__all(all_here)) break;
+        const bool give_up = (++spins > kExchangeSpinLimit) ||
+                             __hip_atomic_load(abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u;
+        if (__any(give_up)) {
+            ok = false;
+            if (lane == 0) {
+                *abort_s = 1;
+                __hip_atomic_store(abort_flag, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+            break;
+        }
+        __builtin_amdgcn_s_sleep(2);
+    }
+#pragma unroll
+    for (int q = 0; q < kMaxWordsPerLane; ++q)
+        if (lane + 64 * q < nw) bits_s[lane + 64 * q] = (unsigned)word[q];
+    return ok;
+}
+
+// row_pull from the CSR in HBM (sparse.c:68-76: the row's edges in CSR order;
+// simple synapses add the weights of spiking columns, PSC synapses advance
+// each edge's trace in place and add w * trace).  Columns are natural neuron
+// ids: bit j % 32 of word j / 32.
+__device__ __forceinline__ double mux_row_pull(int stype, int rs, int re, gptr<const int32_t> col,
+                                               gptr<const double> val, const unsigned* bits,
+                                               gptr<const double> edge_decay, gptr<double> edge_tr) {
+    double s = 0.0;
+    if (stype == ST_SIMPLE) {
+        int e = rs;
+        for (; e + 8 <= re; e += 8) {   // loads of a batch issued before the in-order sum
+            int32_t cj[8];
+            double wv[8];
+#pragma unroll
+            for (int q = 0; q < 8; ++q) { cj[q] = col[e + q]; wv[q] = val[e + q]; }
+#pragma unroll
+            for (int q = 0; q < 8; ++q)
+                if ((bits[cj[q] >> 5] >> (cj[q] & 31)) & 1u) s = s + wv[q];
+        }
+        for (; e < re; ++e) {
+            const int32_t j = col[e];
+            if ((bits[j >> 5] >> (j & 31)) & 1u) s = s + val[e];
+        }
+        return s;
+    }
+    for (int e = rs; e < re; ++e) {
+        const int32_t j = col[e];
+        const unsigned sbit = (bits[j >> 5] >> (j & 31)) & 1u;
+        const double d = edge_decay[e];
+        const double tr = edge_tr[e] * d + (1.0 - d) * (double)sbit;
+        edge_tr[e] = tr;
+        s = s + val[e] * tr;
+    }
+    return s;
+}
+
+constexpr int kMuxNb = 64;
+constexpr int kMuxMaxWords = kMaxWordsPerLane * 64;   // exchange words of one reservoir (n <= 16384)
+constexpr int kMuxBlocks = 2;                          // 16-lag history blocks in flight per lane
+
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) k_mux_gl(
+    const PersistArgs* __restrict__ table, const int2* __restrict__ wg_map) {
+    __shared__ unsigned bits_s[kMuxMaxWords];
+    __shared__ int abort_s;
+    const int2 mw = wg_map[blockIdx.x];
+    const PersistArgs& a = table[mw.x];
+    const int lane = threadIdx.x, g = mw.y;
+    const int n = a.n, L = a.L, wpg = 2, nw = a.G * 2, din = a.din, micro = a.micro, stype = a.stype;
+    const int i0 = g * kMuxNb, nloc = min(kMuxNb, n - i0);
+    const bool own = lane < nloc;
+    const int i = i0 + (own ? lane : nloc - 1);      // lanes past the slice mirror its last neuron
+    gptr<const double> W_in = to_global(a.W_in), u = to_global(a.u), ring_c = to_global((const double*)a.ring);
+    gptr<double> ring = to_global(a.ring);
+    gptr<const double> coef = to_global(a.coef), val = to_global(a.val), decay = to_global(a.edge_decay);
+    gptr<double> edge_tr = to_global(a.edge_tr);
+    gptr<const int32_t> col = to_global(a.col);
+    gptr<const int64_t> rp = to_global(a.rp);
+    gptr<unsigned long long> xch = to_global(a.xch);
+    gptr<unsigned> abort_flag = to_global(a.abort_flag);
+    gptr<double> Vg = to_global(a.V), tpg = to_global(a.tprev);
+    gptr<const uint8_t> S_in = to_global((const uint8_t*)a.S);
+    gptr<uint8_t> S_out = to_global(a.S_out);
+    gptr<double> rec_V = to_global(a.rec_V);
+    gptr<uint8_t> rec_S = to_global(a.rec_S);
+    if (lane == 0) abort_s = 0;
+    long long st0 = a.st0, t0 = a.t0;
+    const long long T = a.T;
+    const double gain = a.gain, dt = a.dt;
+    if (a.clk) {
+        const long long c0 = a.clk[0], c1 = a.clk[1], c2 = a.clk[2];
+        st0 = (long long)(unsigned)__builtin_amdgcn_readfirstlane((int)c0) |
+              ((long long)__builtin_amdgcn_readfirstlane((int)(c0 >> 32)) << 32);
+        t0 = (long long)(unsigned)__builtin_amdgcn_readfirstlane((int)c1) |
+             ((long long)__builtin_amdgcn_readfirstlane((int)(c1 >> 32)) << 32);
+        if (__builtin_amdgcn_readfirstlane((int)c2) & 1) {
+            S_in = to_global((const uint8_t*)a.S_out);
+            S_out = to_global(const_cast<uint8_t*>(a.S));
+        }
+        if (rec_V) rec_V += (size_t)t0 * n;
+        if (rec_S) rec_S += (size_t)t0 * n;
+    }
+    double V = Vg[i], tp = tpg[i];
+    bool spk = S_in[i] != 0;
+    // lanes past the slice pull no row (PSC traces advance once per edge)
+    const int rs = own ? (int)rp[i] : 0, re = own ? (int)rp[i + 1] : 0;
+    double p[7];
+#pragma unroll
+    for (int j = 0; j < 7; ++j) {
+        p[j] = a.np[j];
+        asm volatile("" : "+v"(p[j]));
+    }
+    const unsigned ring_off = (unsigned)i * 8u;   // byte offset of the lane's neuron in a ring slot
+    __syncthreads();
+    long long st = st0;
+    int head = (int)(st % L);
+    int lim = st < L ? (int)st : L - 1;
+    for (long long t = 0; t < T; ++t) {
+        const long long tt = t0 + t;
+        double ext = 0.0;
+        for (int mi = 0; mi < micro; ++mi, ++st) {
+            // GL history of this micro-step (lags 1..lim in ascending order)
+            const int n1 = lim < head ? lim : head;
+            double H = 0.0;
+            H = gl_seg_mux<kMuxBlocks>(H, ring_c, ring_off, coef, 1, head - 1, n1, (size_t)n);
+            H = gl_seg_mux<kMuxBlocks>(H, ring_c, ring_off, coef, n1 + 1, head - n1 - 1 + L, lim - n1, (size_t)n);
+            // spikes of micro-step st-1, CSR rows, input drive
+            const bool aborted = !mux_gather(xch, nw, n, st, st0, S_in, bits_s, abort_flag, &abort_s, lane);
+            __syncthreads();
+            if (__builtin_amdgcn_readfirstlane(abort_s) || aborted) break;
+            const double rec = mux_row_pull(stype, rs, re, col, val, bits_s, decay, edge_tr);
+            if (mi == 0) {
+                double su = 0.0;
+                for (int j = 0; j < din; ++j) su = su + W_in[(size_t)i * din + j] * u[tt * din + j];
+                ext = gain * su;
+            }
+            // neuron update (flif_gl.c:50-100)
+            const double x = ext + rec;
+            spk = false;
+            if (tp > p[6]) {
+                const int sp = head == 0 ? L - 1 : head - 1;
+                const double Vp = ring[(size_t)sp * n + i];
+                const double rhs = (-(Vp - p[2]) / p[3]) + x + p[5];
+                V = p[4] * rhs - H;
+                if (V >= p[0]) { V = p[1]; spk = true; tp = 0.0; }
+            } else {
+                V = p[1];
+            }
+            // publish: words 2g and 2g+1 of the parity slot (tag st + 1, 32 spike bits each)
+            const unsigned long long bal = __ballot(own && spk);
+            if (lane < wpg) {
+                const unsigned bits = (unsigned)(bal >> (32 * lane));
+                const unsigned long long word = ((unsigned long long)(unsigned)(st + 1) << 32) | bits;
+                __hip_atomic_store(&xch[(size_t)(st & 1) * nw + (size_t)g * wpg + lane], word, __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+            }
+            if (own) {
+                ring[(size_t)head * n + i] = V;
+                tp = tp + dt;
+                if (mi == micro - 1) {
+                    if (rec_V) rec_V[(size_t)t * n + i] = V;
+                    if (rec_S) rec_S[(size_t)t * n + i] = spk ? 1 : 0;
+                }
+            }
+            head = head + 1 == L ? 0 : head + 1;
+            if (lim < L - 1) ++lim;
+            __syncthreads();   // bits_s is rewritten by the next gather
+        }
+        if (__builtin_amdgcn_readfirstlane(abort_s)) break;
+    }
+    if (own) {
+        Vg[i] = V;
+        S_out[i] = spk ? 1 : 0;
+        tpg[i] = tp;
+    }
+}
+
 }  // namespace sdev
 
 struct spires_dev_timeblock {
@@ -1483,6 +1758,8 @@ struct spires_dev_timeblock {
     std::vector<const double*> u_at;
     std::vector<double*> recv_at;
     std::vector<uint8_t*> recs_at;
+    std::vector<sdev::Plan> plans;          // per reservoir, launch order
+    std::vector<sdev::PersistArgs> args;    // host copy of the table
 };
 
 extern "C" void spires_dev_timeblock_destroy(spires_dev_timeblock* tb) {
@@ -1502,17 +1779,26 @@ extern "C" int spires_dev_timeblock_create(spires_dev* const* gs, int count, lon
     using namespace sdev;
     if (!gs || count < 1 || K < 1 || !out) return SPIRES_DEV_EINVAL;
     *out = nullptr;
-    if (flags & ~(SPIRES_DEV_REC_STATE | SPIRES_DEV_REC_RASTER | SPIRES_DEV_REC_APPEND)) {
+    if (flags & ~(SPIRES_DEV_REC_STATE | SPIRES_DEV_REC_RASTER | SPIRES_DEV_REC_APPEND | SPIRES_DEV_TB_ONE_LAUNCH)) {
         set_error("time blocks record states and rasters only");
         return SPIRES_DEV_EINVAL;
     }
+    const bool one = (flags & SPIRES_DEV_TB_ONE_LAUNCH) != 0;
     const int ncu = num_cus();
     std::vector<Plan> plans(count);
     for (int r = 0; r < count; ++r) {
         spires_dev* g = gs[r];
         if (!g) return SPIRES_DEV_EINVAL;
         if (const int rc = settle(g)) return rc;
-        if (g->mode != SPIRES_DEV_MODE_EXACT || g->path == 1 || !persist_plan(g, ncu, &plans[r], true) || !plans[r].tp) {
+        if (one) {
+            // the one-launch kernel: FLIF-GL, CSR, simple or PSC synapses, any history length
+            if (g->mode != SPIRES_DEV_MODE_EXACT || g->path == 1 || g->backend != BK_SPARSE || g->ntype != NT_FLIF_GL ||
+                !(g->stype == ST_SIMPLE || g->stype == ST_PSC_HET) || g->L < 2) {
+                set_error("one-launch time blocks need exact-mode FLIF-GL reservoirs with CSR weights");
+                return SPIRES_DEV_EINVAL;
+            }
+        } else if (g->mode != SPIRES_DEV_MODE_EXACT || g->path == 1 || !persist_plan(g, ncu, &plans[r], true) ||
+                   !plans[r].tp) {
             set_error("time blocks need exact-mode FLIF-GL reservoirs that fit the product-wave kernel");
             return SPIRES_DEV_EINVAL;
         }
@@ -1522,13 +1808,37 @@ extern "C" int spires_dev_timeblock_create(spires_dev* const* gs, int count, lon
         for (int q = 0; q < r; ++q)
             if (gs[q] == g) { set_error("a reservoir appears twice in a time block"); return SPIRES_DEV_EINVAL; }
     }
+    if (one) {
+        int total = 0;
+        for (int r = 0; r < count; ++r) {
+            if (gs[r]->n > kMuxMaxWords * 32) { set_error("one-launch time blocks take reservoirs of at most 16384 neurons"); return SPIRES_DEV_EINVAL; }
+            total += (gs[r]->n + kMuxNb - 1) / kMuxNb;
+        }
+        int per_cu = 0;
+        SDEV_CK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k_mux_gl, 64, 0));
+        if ((long long)per_cu * ncu < total) {
+            set_error("one-launch time block: " + std::to_string(total) + " slices do not fit the " +
+                      std::to_string((long long)per_cu * ncu) + " resident workgroups");
+            return SPIRES_DEV_EINVAL;
+        }
+    }
     // pack launches per neurons-per-workgroup class, at most one workgroup per CU.
     // A launch lasts as long as its slowest member, whose step time grows with its
     // history length: reservoirs are placed longest history first (then most
     // workgroups first) into the first launch with room, so a shorter history only
     // fills room next to longer ones and never lengthens a launch.
     std::vector<std::vector<int>> launches;
+    if (one) {   // every reservoir in one launch of 64-neuron slices
+        launches.emplace_back();
+        for (int r = 0; r < count; ++r) {
+            launches[0].push_back(r);
+            plans[r] = Plan();
+            plans[r].nb = kMuxNb;
+            plans[r].G = (gs[r]->n + kMuxNb - 1) / kMuxNb;
+        }
+    }
     for (int nb : {32, 16}) {
+        if (one) break;
         std::vector<int> idx;
         for (int r = 0; r < count; ++r)
             if (plans[r].nb == nb) idx.push_back(r);
@@ -1589,12 +1899,13 @@ extern "C" int spires_dev_timeblock_create(spires_dev* const* gs, int count, lon
             tb->u_at.push_back(g->u_series);
             tb->recv_at.push_back(g->rec_state);
             tb->recs_at.push_back(g->rec_raster);
+            tb->plans.push_back(pl);
             l = std::max(l, pl.lds);
         }
         lds.push_back(l);
         const int nb = plans[L[0]].nb;
         nbs.push_back(nb);
-        (nb == 32 ? lds32 : lds16) = std::max(nb == 32 ? lds32 : lds16, l);
+        if (nb != kMuxNb) (nb == 32 ? lds32 : lds16) = std::max(nb == 32 ? lds32 : lds16, l);
     }
     if ((rc = dalloc(&tb->table, table.size())) || (rc = dalloc(&tb->map, map.size())) ||
         (rc = dalloc(&tb->items, items.size())))
@@ -1607,6 +1918,7 @@ extern "C" int spires_dev_timeblock_create(spires_dev* const* gs, int count, lon
     if (e == hipSuccess && lds32) e = hipFuncSetAttribute((const void*)k_persist_tp_batch<32>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds32);
     if (e == hipSuccess && lds16) e = hipFuncSetAttribute((const void*)k_persist_tp_batch<16>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds16);
     if (e != hipSuccess) { set_error(hipGetErrorString(e)); return fail(SPIRES_DEV_EINTERNAL); }
+    tb->args = table;
     // capture one block: each launch (cooperative, so that the runtime checks that
     // all its workgroups can be resident at once, as the spike exchange needs),
     // then the clock advance of its reservoirs
@@ -1617,8 +1929,12 @@ extern "C" int spires_dev_timeblock_create(spires_dev* const* gs, int count, lon
         const PersistArgs* tab = tb->table;
         const int2* mp = tb->map + map_off[q];
         void* kargs[] = {(void*)&tab, (void*)&mp};
-        const void* fn = nbs[q] == 32 ? (const void*)k_persist_tp_batch<32> : (const void*)k_persist_tp_batch<16>;
-        e = hipLaunchCooperativeKernel(fn, dim3(G), dim3(kTpThreads), kargs, (unsigned)lds[q], tb->stream);
+        if (nbs[q] == kMuxNb)
+            e = hipLaunchCooperativeKernel((const void*)k_mux_gl, dim3(G), dim3(64), kargs, 0, tb->stream);
+        else
+            e = hipLaunchCooperativeKernel(nbs[q] == 32 ? (const void*)k_persist_tp_batch<32>
+                                                        : (const void*)k_persist_tp_batch<16>,
+                                           dim3(G), dim3(kTpThreads), kargs, (unsigned)lds[q], tb->stream);
         if (e != hipSuccess) break;
         hipLaunchKernelGGL(k_timeblock_advance, dim3(1), dim3(64 * ((cnt + 63) / 64)), 0, tb->stream,
                            (const TbItem*)(tb->items + item_off[q]), cnt);
@@ -1661,6 +1977,13 @@ extern "C" int spires_dev_timeblock_run(spires_dev_timeblock* tb, long long bloc
             return SPIRES_DEV_EINVAL;
         }
         if (!g->ring64_valid) { set_error("the FP64 history is stale (fast-mode run after the time block was made)"); return SPIRES_DEV_EINVAL; }
+        // a run outside the block may have grown (reallocated) the exchange words:
+        // size them for this block's plan again and point the table at them
+        if (const int rc = prepare_launch(g, tb->plans[r])) return rc;
+        if (tb->args[r].xch != g->xch) {
+            tb->args[r].xch = g->xch;
+            SDEV_CK(hipMemcpyAsync(&tb->table[r].xch, &tb->args[r].xch, sizeof(g->xch), hipMemcpyHostToDevice, tb->stream));
+        }
         SDEV_CK(hipStreamSynchronize(g->stream));
         clk[4 * r + 0] = g->step;
         clk[4 * r + 1] = (long long)g->msteps;

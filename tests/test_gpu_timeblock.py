@@ -44,11 +44,16 @@ def _same(a, b, rows):
     return np.array_equal(Ra, Rb) and np.array_equal(_bits(Xa), _bits(Xb))
 
 
-def test_timeblock_replays_match_eager_runs_and_oracle():
+@pytest.mark.parametrize("one", [False, True], ids=["packed", "one_launch"])
+def test_timeblock_replays_match_eager_runs_and_oracle(one):
+    """Packed product-wave launches, or every reservoir in one launch of
+    64-neuron one-wave slices (TB_ONE_LAUNCH): both equal the eager batched runs
+    (the product-wave kernel) and the oracle bit for bit."""
+    print(f"case: time block, {'one launch' if one else 'packed launches'}", flush=True)
     us = [np.random.default_rng(40 + k).uniform(0, 0.5, (T_ALL, 1)) for k in range(len(SPECS))]
     A = [_make(s, u) for s, u in zip(SPECS, us)]     # graph
     B = [_make(s, u) for s, u in zip(SPECS, us)]     # eager
-    tb = sa.TimeBlock(A, 20, FLAGS)
+    tb = sa.TimeBlock(A, 20, FLAGS | (sa.TB_ONE_LAUNCH if one else 0))
     assert tb.launches == 1
     tb.run(2)
     sa.run_resident_batch(B, 0, 40, FLAGS | sa.REC_APPEND)
@@ -83,6 +88,50 @@ def test_timeblock_replays_match_eager_runs_and_oracle():
     A[2].reserve_records(2 * T_ALL, FLAGS)
     with pytest.raises(sa.SpiresError):
         tb.run(1)
+    tb.close()
+    for r in A + B:
+        r.close()
+
+
+def test_one_launch_time_block_long_histories_and_psc():
+    """The one-launch kernel on reservoirs the product-wave batch does not take
+    together (T_mem 1024 beside 64, a PSC synapse) against single-reservoir
+    runs of the product-wave kernels and the oracle, bit for bit."""
+    print("case: one-launch time block, long histories and a PSC synapse", flush=True)
+    specs = [(700, 1024, 0.7, "random", 0.05, 2.0, 7), (130, 64, 0.9, "small_world", 0.1, 1.5, 8)]
+    T = 1500
+    us = [np.random.default_rng(50 + k).uniform(0, 0.5, (T, 1)) for k in range(2)]
+
+    def make(spec, u, synapse="simple"):
+        sa.srand(spec[-1])
+        kw = _kw(spec)
+        if synapse != "simple":
+            kw.update(synapse=synapse, syn_params=[0.5, 8.0])
+        r = sa.Reservoir(sa.make_config(spec[0], **kw))
+        r.upload_inputs(u)
+        r.reserve_records(T, FLAGS)
+        return r
+
+    A = [make(specs[0], us[0]), make(specs[1], us[1], "psc_heterogeneous")]
+    B = [make(specs[0], us[0]), make(specs[1], us[1], "psc_heterogeneous")]
+    tb = sa.TimeBlock(A, 400, FLAGS | sa.TB_ONE_LAUNCH)
+    assert tb.launches == 1
+    tb.run(1)
+    # a single-reservoir run between replays takes more exchange words than the
+    # block's 64-neuron slices (the block must follow the reallocated words)
+    A[1].run_resident(400, 300, FLAGS | sa.REC_APPEND)
+    tb.run(2)
+    B[0].run_resident(0, 1200, FLAGS | sa.REC_APPEND)
+    B[1].run_resident(0, 1500, FLAGS | sa.REC_APPEND)
+    assert _same(A[0], B[0], 1200) and _same(A[1], B[1], 1500)
+    for a, b in zip(A, B):
+        assert np.array_equal(_bits(a.state()), _bits(b.state()))
+    spec, u = specs[0], us[0]
+    po.srand(spec[-1])
+    o = po.OracleReservoir(po.make_config(spec[0], **_kw(spec)))
+    Xo, Ro, _ = o.run_record(u[:600], 600)
+    X, R, _ = A[0].read_records(0, 600, states=True, raster=True)
+    assert Ro.sum() > 0 and np.array_equal(R, Ro) and np.array_equal(_bits(X), _bits(Xo))
     tb.close()
     for r in A + B:
         r.close()
