@@ -7,6 +7,11 @@
 // in the same 64-bit word as the tag, so no fence is needed.  Because each
 // workgroup reads all words every step, no workgroup is ever more than one
 // step ahead of another, so two parity slots suffice.
+//
+// Each word sits in a 64-byte line of its own (kXchStride words apart): every
+// workgroup polls every word, and words packed into a few lines make those
+// lines a hot spot for the far-memory loads (MI355X, 256 workgroups, exchange
+// alone: 2.13 us per round with spread words, 2.2-2.5 us with packed words).
 #pragma once
 #include "dev_internal.hpp"
 
@@ -14,14 +19,15 @@ namespace sdev {
 
 constexpr int kMaxWordsPerLane = 8;     // exchange words polled per lane (G * wpg <= 512)
 constexpr long long kExchangeSpinLimit = 1ll << 22;   // ~seconds of polling before a launch aborts
+constexpr int kXchStride = 8;           // u64 words from one exchange word to the next
 
 __device__ __forceinline__ void publish_spikes(unsigned long long* xch, int nw, int g, int wpg, long long st,
                                                int lane, unsigned long long ballot_mask) {
     if (lane < wpg) {
         const unsigned bits = (unsigned)(ballot_mask >> (32 * lane));
         const unsigned long long word = ((unsigned long long)(unsigned)(st + 1) << 32) | bits;
-        __hip_atomic_store(xch + (size_t)(st & 1) * nw + (size_t)g * wpg + lane, word, __ATOMIC_RELAXED,
-                           __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(xch + ((size_t)(st & 1) * nw + (size_t)g * wpg + lane) * kXchStride, word,
+                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
 }
 
@@ -30,6 +36,9 @@ __device__ __forceinline__ void publish_spikes(unsigned long long* xch, int nw, 
 // launch; neuron r of workgroup gg is gg * nb + r, or gg + ileave * r when the
 // workgroups interleave neurons (ileave = workgroup count).  Returns false
 // (after raising the abort flags) on timeout or when another workgroup aborted.
+// Each poll reloads all nq = ceil(nw / 64) words of a lane (no per-word
+// branches); one poll is in flight at a time, so none is left outstanding for
+// the barrier that follows.  The shared abort flag is read every 64 polls.
 __device__ __forceinline__ bool gather_spikes(const unsigned long long* xch, int nw, int nb, int wpg, int n,
                                               long long st, long long st0, const uint8_t* S, unsigned* bits_s,
                                               unsigned* abort_flag, int* abort_s, int lane, int ileave = 0) {
@@ -46,46 +55,50 @@ __device__ __forceinline__ bool gather_spikes(const unsigned long long* xch, int
         }
         return true;
     }
-    const unsigned long long* slot = xch + (size_t)((st - 1) & 1) * nw;
+    const unsigned long long* slot = xch + (size_t)((st - 1) & 1) * nw * kXchStride;
     const unsigned tag = (unsigned)st;
-    unsigned long long word[kMaxWordsPerLane];
-    bool ready[kMaxWordsPerLane];
+    const int nq = (nw + 63) >> 6;                     // uniform
+    unsigned word[kMaxWordsPerLane];
+    unsigned rdy = 0;                                  // bit q: word q of this lane seen (or absent)
 #pragma unroll
-    for (int q = 0; q < kMaxWordsPerLane; ++q) { ready[q] = lane + 64 * q >= nw; word[q] = 0; }
+    for (int q = 0; q < kMaxWordsPerLane; ++q) {
+        word[q] = 0;
+        if (q >= nq || lane + 64 * q >= nw) rdy |= 1u << q;
+    }
+    const unsigned full = (1u << kMaxWordsPerLane) - 1u;
     long long spins = 0;
     bool ok = true;
     while (true) {
+        unsigned long long w[kMaxWordsPerLane];
 #pragma unroll
         for (int q = 0; q < kMaxWordsPerLane; ++q)
-            if (!ready[q]) word[q] = __hip_atomic_load(&slot[lane + 64 * q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        bool all_here = true;
+            if (q < nq)
+                w[q] = __hip_atomic_load(&slot[(size_t)min(lane + 64 * q, nw - 1) * kXchStride], __ATOMIC_RELAXED,
+                                         __HIP_MEMORY_SCOPE_AGENT);
 #pragma unroll
-        for (int q = 0; q < kMaxWordsPerLane; ++q) {
-            if (!ready[q]) ready[q] = (unsigned)(word[q] >> 32) == tag;
-            all_here = all_here && ready[q];
-        }
-        if (__all(all_here)) break;
-        const bool give_up = (++spins > kExchangeSpinLimit) ||
-                             __hip_atomic_load(abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u;
-        if (__any(give_up)) {
-            ok = false;
-            if (lane == 0) {
-                *abort_s = 1;
-                __hip_atomic_store(abort_flag, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        for (int q = 0; q < kMaxWordsPerLane; ++q)
+            if (q < nq && (unsigned)(w[q] >> 32) == tag) {
+                word[q] = (unsigned)w[q];
+                rdy |= 1u << q;
             }
-            break;
+        if (__all(rdy == full)) break;
+        if ((++spins & 63) == 0) {
+            const bool give_up = spins > kExchangeSpinLimit ||
+                                 __hip_atomic_load(abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u;
+            if (__any(give_up)) {
+                ok = false;
+                if (lane == 0) {
+                    *abort_s = 1;
+                    __hip_atomic_store(abort_flag, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                }
+                break;
+            }
         }
-        // Every workgroup polls the same exchange words with far-memory loads.
-        // With 128 or more words (and as many polling workgroups), polling about
-        // every 512 cycles instead of every 64 keeps them from becoming a hot
-        // spot (FLIF-GL N=4096, L=512: 3.5 us per step, not 3.9); smaller
-        // reservoirs keep the short interval.
-        if (nw >= 128) __builtin_amdgcn_s_sleep(8);
-        else __builtin_amdgcn_s_sleep(1);
+        __builtin_amdgcn_s_sleep(1);
     }
 #pragma unroll
     for (int q = 0; q < kMaxWordsPerLane; ++q)
-        if (lane + 64 * q < nw) bits_s[lane + 64 * q] = (unsigned)word[q];
+        if (q < nq && lane + 64 * q < nw) bits_s[lane + 64 * q] = word[q];
     return ok;
 }
 

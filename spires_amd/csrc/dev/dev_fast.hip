@@ -483,12 +483,12 @@ int launch_fast(spires_dev* g, const double* u_series, long long t0, long long T
     K = (K / 16) * 16;
     if (K < 16) K = 16;
     while (K > g->L - 1 && K > 16) K -= 16;
-    if (g->xch_words < 2 * nw || g->xch_plan_G != -G) {
-        if (g->xch_words < 2 * nw) {
+    if (g->xch_words < 2 * nw * kXchStride || g->xch_plan_G != -G) {
+        if (g->xch_words < 2 * nw * kXchStride) {
             dfree(g->xch);
-            int rc = dalloc(&g->xch, 2 * nw);
+            int rc = dalloc(&g->xch, 2 * nw * kXchStride);
             if (rc) return rc;
-            g->xch_words = 2 * nw;
+            g->xch_words = 2 * nw * kXchStride;
         }
         SDEV_CK(hipMemsetAsync(g->xch, 0, g->xch_words * 8, g->stream));
         g->xch_plan_G = -G;   // negative: plan of the fast kernel

@@ -43,6 +43,7 @@ struct PersistArgs {
     int ne_lds;                   // LDS capacity for CSR edges per workgroup (0: read CSR from HBM)
     int rpm, cw;                  // row pull: 0 edge list, 1 spike-word bitmaps (cw words per scan part)
     int win;                      // k_persist_tp: product window in 32-lag chunks (0: whole rows)
+    int hl;                       // k_persist_tp: the workgroup's history rows kept in LDS (whole rows only)
     long long st0;                // micro-step index since reset at launch start
     long long t0, T;              // macro steps [t0, t0+T) of the resident input series
     double gain, dt;
@@ -446,6 +447,13 @@ __global__ void __launch_bounds__(kPersistThreads) __attribute__((amdgpu_waves_p
 constexpr int kTpThreads = 384;
 constexpr int kTpChunk = 32;
 constexpr int kTpProducers = 4;
+// Wave roles: 0 the chain, kTpXchWave the exchange, the other four the products.
+// Waves w and w+4 share a SIMD.  The chain issues dependent adds back to back,
+// so a wave beside it gets few issue slots: the exchange wave sits there rather
+// than a product wave, whose last chunks would then finish after the chain.
+constexpr int kTpXchWave = 4;
+constexpr int kTpLoadDepth = 8;      // ring-load buffers of a product wave (chunks in flight + 1)
+constexpr int kRegEdges = 12;        // row pull from registers (16 neurons per workgroup): rows of at most 12 edges
 constexpr long long kTpSpinLimit = 1ll << 24;
 constexpr int kBitmapBatch = 16;     // weights loaded per batch of the bitmap pull from HBM (half from LDS)
 
@@ -459,6 +467,229 @@ __device__ __forceinline__ void lds_counter_store(unsigned* f, unsigned v) {
 // Product-row stride in doubles: every chunk of 32 lags starting at lag 2 fits,
 // and 2*PS = 4 (mod 32) dwords keeps the rows' 16-byte reads conflict-free.
 __host__ __device__ __forceinline__ int tp_row_stride(int L) { return 32 * ((L - 2 + 31) / 32) + 2; }
+// History row in LDS (k_persist_tp, whole rows): slot s of the ring at entry
+// L - 1 - s, so ascending lags are ascending entries, and the entries past L - 1
+// repeat entries 0, 1, ... (mod L), so that up to five consecutive lags read
+// from consecutive entries without a wrap.  The stride is odd: with rows a
+// multiple of 8 doubles apart the product waves' reads of 16 rows pile onto a
+// few LDS banks (N=4096, L=512: 4.1 us per step at stride L+8, 3.23 at L+7).
+__host__ __device__ __forceinline__ int tp_hist_stride(int L) { return (L + 5) | 1; }
+
+// The chain of one product row: h += P[k] for k = 2 .. 2 + 32 * nch - 1 in
+// ascending order, one dependent v_add_f64 per lag (flif_gl.c:66-72); entries
+// past the step's last lag are +0.0, which leaves h unchanged (h is never -0.0:
+// it starts as 0.0 + c_1 V).  A wave issues nothing else while one of its FP64
+// adds is in flight, so every other instruction of the chain wave lengthens the
+// chain: the 16-byte reads of lag pairs go out in bursts of 8 right after a wait
+// (a read issued between two adds costs about twice as much as one in a burst),
+// 14 pairs ahead of the adds, into a 32-pair ring in v[64:191].  The waits
+// count reads only, which is conservative when a counter store is outstanding,
+// and at most 15 LDS operations are ever in flight.  The last bursts read up to
+// 224 bytes past the row (the next row, or the coefficient table after the last
+// one) and are waited for before the statement ends.  After chunk c (32 lags)
+// the LDS counter at byte address ca is set to cv + c, as the producers expect.
+// va is the byte address of P[2] in LDS (16-byte aligned).
+__device__ __forceinline__ double chain_chunks(double h, unsigned va, int nch, unsigned ca, unsigned cv) {
+    asm volatile(
+        "s_waitcnt lgkmcnt(0)\n\t"
+        "ds_read_b128 v[64:67], %[va] offset:0\n\t"
+        "ds_read_b128 v[68:71], %[va] offset:16\n\t"
+        "ds_read_b128 v[72:75], %[va] offset:32\n\t"
+        "ds_read_b128 v[76:79], %[va] offset:48\n\t"
+        "ds_read_b128 v[80:83], %[va] offset:64\n\t"
+        "ds_read_b128 v[84:87], %[va] offset:80\n\t"
+        "ds_read_b128 v[88:91], %[va] offset:96\n\t"
+        "ds_read_b128 v[92:95], %[va] offset:112\n\t"
+        "ds_read_b128 v[96:99], %[va] offset:128\n\t"
+        "ds_read_b128 v[100:103], %[va] offset:144\n\t"
+        "ds_read_b128 v[104:107], %[va] offset:160\n\t"
+        "ds_read_b128 v[108:111], %[va] offset:176\n\t"
+        "ds_read_b128 v[112:115], %[va] offset:192\n\t"
+        "ds_read_b128 v[116:119], %[va] offset:208\n\t"
+        "L_chain_loop_%=:\n\t"
+        "s_cmp_lt_u32 %[n], 2\n\t"
+        "s_cbranch_scc1 L_chain_half_%=\n\t"
+        "s_waitcnt lgkmcnt(6)\n\t"
+        "ds_read_b128 v[120:123], %[va] offset:224\n\t"
+        "ds_read_b128 v[124:127], %[va] offset:240\n\t"
+        "ds_read_b128 v[128:131], %[va] offset:256\n\t"
+        "ds_read_b128 v[132:135], %[va] offset:272\n\t"
+        "ds_read_b128 v[136:139], %[va] offset:288\n\t"
+        "ds_read_b128 v[140:143], %[va] offset:304\n\t"
+        "ds_read_b128 v[144:147], %[va] offset:320\n\t"
+        "ds_read_b128 v[148:151], %[va] offset:336\n\t"
+        "v_add_f64 %[h], %[h], v[64:65]\n\t"
+        "v_add_f64 %[h], %[h], v[66:67]\n\t"
+        "v_add_f64 %[h], %[h], v[68:69]\n\t"
+        "v_add_f64 %[h], %[h], v[70:71]\n\t"
+        "v_add_f64 %[h], %[h], v[72:73]\n\t"
+        "v_add_f64 %[h], %[h], v[74:75]\n\t"
+        "v_add_f64 %[h], %[h], v[76:77]\n\t"
+        "v_add_f64 %[h], %[h], v[78:79]\n\t"
+        "v_add_f64 %[h], %[h], v[80:81]\n\t"
+        "v_add_f64 %[h], %[h], v[82:83]\n\t"
+        "v_add_f64 %[h], %[h], v[84:85]\n\t"
+        "v_add_f64 %[h], %[h], v[86:87]\n\t"
+        "v_add_f64 %[h], %[h], v[88:89]\n\t"
+        "v_add_f64 %[h], %[h], v[90:91]\n\t"
+        "v_add_f64 %[h], %[h], v[92:93]\n\t"
+        "v_add_f64 %[h], %[h], v[94:95]\n\t"
+        "s_waitcnt lgkmcnt(6)\n\t"
+        "ds_read_b128 v[152:155], %[va] offset:352\n\t"
+        "ds_read_b128 v[156:159], %[va] offset:368\n\t"
+        "ds_read_b128 v[160:163], %[va] offset:384\n\t"
+        "ds_read_b128 v[164:167], %[va] offset:400\n\t"
+        "ds_read_b128 v[168:171], %[va] offset:416\n\t"
+        "ds_read_b128 v[172:175], %[va] offset:432\n\t"
+        "ds_read_b128 v[176:179], %[va] offset:448\n\t"
+        "ds_read_b128 v[180:183], %[va] offset:464\n\t"
+        "v_add_f64 %[h], %[h], v[96:97]\n\t"
+        "v_add_f64 %[h], %[h], v[98:99]\n\t"
+        "v_add_f64 %[h], %[h], v[100:101]\n\t"
+        "v_add_f64 %[h], %[h], v[102:103]\n\t"
+        "v_add_f64 %[h], %[h], v[104:105]\n\t"
+        "v_add_f64 %[h], %[h], v[106:107]\n\t"
+        "v_add_f64 %[h], %[h], v[108:109]\n\t"
+        "v_add_f64 %[h], %[h], v[110:111]\n\t"
+        "v_add_f64 %[h], %[h], v[112:113]\n\t"
+        "v_add_f64 %[h], %[h], v[114:115]\n\t"
+        "v_add_f64 %[h], %[h], v[116:117]\n\t"
+        "v_add_f64 %[h], %[h], v[118:119]\n\t"
+        "v_add_f64 %[h], %[h], v[120:121]\n\t"
+        "v_add_f64 %[h], %[h], v[122:123]\n\t"
+        "v_add_f64 %[h], %[h], v[124:125]\n\t"
+        "v_add_f64 %[h], %[h], v[126:127]\n\t"
+        "ds_write_b32 %[ca], %[cv]\n\t"
+        "v_add_u32 %[cv], 1, %[cv]\n\t"
+        "s_waitcnt lgkmcnt(6)\n\t"
+        "ds_read_b128 v[184:187], %[va] offset:480\n\t"
+        "ds_read_b128 v[188:191], %[va] offset:496\n\t"
+        "ds_read_b128 v[64:67], %[va] offset:512\n\t"
+        "ds_read_b128 v[68:71], %[va] offset:528\n\t"
+        "ds_read_b128 v[72:75], %[va] offset:544\n\t"
+        "ds_read_b128 v[76:79], %[va] offset:560\n\t"
+        "ds_read_b128 v[80:83], %[va] offset:576\n\t"
+        "ds_read_b128 v[84:87], %[va] offset:592\n\t"
+        "v_add_f64 %[h], %[h], v[128:129]\n\t"
+        "v_add_f64 %[h], %[h], v[130:131]\n\t"
+        "v_add_f64 %[h], %[h], v[132:133]\n\t"
+        "v_add_f64 %[h], %[h], v[134:135]\n\t"
+        "v_add_f64 %[h], %[h], v[136:137]\n\t"
+        "v_add_f64 %[h], %[h], v[138:139]\n\t"
+        "v_add_f64 %[h], %[h], v[140:141]\n\t"
+        "v_add_f64 %[h], %[h], v[142:143]\n\t"
+        "v_add_f64 %[h], %[h], v[144:145]\n\t"
+        "v_add_f64 %[h], %[h], v[146:147]\n\t"
+        "v_add_f64 %[h], %[h], v[148:149]\n\t"
+        "v_add_f64 %[h], %[h], v[150:151]\n\t"
+        "v_add_f64 %[h], %[h], v[152:153]\n\t"
+        "v_add_f64 %[h], %[h], v[154:155]\n\t"
+        "v_add_f64 %[h], %[h], v[156:157]\n\t"
+        "v_add_f64 %[h], %[h], v[158:159]\n\t"
+        "s_waitcnt lgkmcnt(6)\n\t"
+        "ds_read_b128 v[88:91], %[va] offset:608\n\t"
+        "ds_read_b128 v[92:95], %[va] offset:624\n\t"
+        "ds_read_b128 v[96:99], %[va] offset:640\n\t"
+        "ds_read_b128 v[100:103], %[va] offset:656\n\t"
+        "ds_read_b128 v[104:107], %[va] offset:672\n\t"
+        "ds_read_b128 v[108:111], %[va] offset:688\n\t"
+        "ds_read_b128 v[112:115], %[va] offset:704\n\t"
+        "ds_read_b128 v[116:119], %[va] offset:720\n\t"
+        "v_add_f64 %[h], %[h], v[160:161]\n\t"
+        "v_add_f64 %[h], %[h], v[162:163]\n\t"
+        "v_add_f64 %[h], %[h], v[164:165]\n\t"
+        "v_add_f64 %[h], %[h], v[166:167]\n\t"
+        "v_add_f64 %[h], %[h], v[168:169]\n\t"
+        "v_add_f64 %[h], %[h], v[170:171]\n\t"
+        "v_add_f64 %[h], %[h], v[172:173]\n\t"
+        "v_add_f64 %[h], %[h], v[174:175]\n\t"
+        "v_add_f64 %[h], %[h], v[176:177]\n\t"
+        "v_add_f64 %[h], %[h], v[178:179]\n\t"
+        "v_add_f64 %[h], %[h], v[180:181]\n\t"
+        "v_add_f64 %[h], %[h], v[182:183]\n\t"
+        "v_add_f64 %[h], %[h], v[184:185]\n\t"
+        "v_add_f64 %[h], %[h], v[186:187]\n\t"
+        "v_add_f64 %[h], %[h], v[188:189]\n\t"
+        "v_add_f64 %[h], %[h], v[190:191]\n\t"
+        "ds_write_b32 %[ca], %[cv]\n\t"
+        "v_add_u32 %[cv], 1, %[cv]\n\t"
+        "v_add_u32 %[va], 0x200, %[va]\n\t"
+        "s_sub_u32 %[n], %[n], 2\n\t"
+        "s_branch L_chain_loop_%=\n\t"
+        "L_chain_half_%=:\n\t"
+        "s_cmp_eq_u32 %[n], 0\n\t"
+        "s_cbranch_scc1 L_chain_end_%=\n\t"
+        "s_waitcnt lgkmcnt(6)\n\t"
+        "ds_read_b128 v[120:123], %[va] offset:224\n\t"
+        "ds_read_b128 v[124:127], %[va] offset:240\n\t"
+        "ds_read_b128 v[128:131], %[va] offset:256\n\t"
+        "ds_read_b128 v[132:135], %[va] offset:272\n\t"
+        "ds_read_b128 v[136:139], %[va] offset:288\n\t"
+        "ds_read_b128 v[140:143], %[va] offset:304\n\t"
+        "ds_read_b128 v[144:147], %[va] offset:320\n\t"
+        "ds_read_b128 v[148:151], %[va] offset:336\n\t"
+        "v_add_f64 %[h], %[h], v[64:65]\n\t"
+        "v_add_f64 %[h], %[h], v[66:67]\n\t"
+        "v_add_f64 %[h], %[h], v[68:69]\n\t"
+        "v_add_f64 %[h], %[h], v[70:71]\n\t"
+        "v_add_f64 %[h], %[h], v[72:73]\n\t"
+        "v_add_f64 %[h], %[h], v[74:75]\n\t"
+        "v_add_f64 %[h], %[h], v[76:77]\n\t"
+        "v_add_f64 %[h], %[h], v[78:79]\n\t"
+        "v_add_f64 %[h], %[h], v[80:81]\n\t"
+        "v_add_f64 %[h], %[h], v[82:83]\n\t"
+        "v_add_f64 %[h], %[h], v[84:85]\n\t"
+        "v_add_f64 %[h], %[h], v[86:87]\n\t"
+        "v_add_f64 %[h], %[h], v[88:89]\n\t"
+        "v_add_f64 %[h], %[h], v[90:91]\n\t"
+        "v_add_f64 %[h], %[h], v[92:93]\n\t"
+        "v_add_f64 %[h], %[h], v[94:95]\n\t"
+        "s_waitcnt lgkmcnt(6)\n\t"
+        "ds_read_b128 v[152:155], %[va] offset:352\n\t"
+        "ds_read_b128 v[156:159], %[va] offset:368\n\t"
+        "ds_read_b128 v[160:163], %[va] offset:384\n\t"
+        "ds_read_b128 v[164:167], %[va] offset:400\n\t"
+        "ds_read_b128 v[168:171], %[va] offset:416\n\t"
+        "ds_read_b128 v[172:175], %[va] offset:432\n\t"
+        "ds_read_b128 v[176:179], %[va] offset:448\n\t"
+        "ds_read_b128 v[180:183], %[va] offset:464\n\t"
+        "v_add_f64 %[h], %[h], v[96:97]\n\t"
+        "v_add_f64 %[h], %[h], v[98:99]\n\t"
+        "v_add_f64 %[h], %[h], v[100:101]\n\t"
+        "v_add_f64 %[h], %[h], v[102:103]\n\t"
+        "v_add_f64 %[h], %[h], v[104:105]\n\t"
+        "v_add_f64 %[h], %[h], v[106:107]\n\t"
+        "v_add_f64 %[h], %[h], v[108:109]\n\t"
+        "v_add_f64 %[h], %[h], v[110:111]\n\t"
+        "v_add_f64 %[h], %[h], v[112:113]\n\t"
+        "v_add_f64 %[h], %[h], v[114:115]\n\t"
+        "v_add_f64 %[h], %[h], v[116:117]\n\t"
+        "v_add_f64 %[h], %[h], v[118:119]\n\t"
+        "v_add_f64 %[h], %[h], v[120:121]\n\t"
+        "v_add_f64 %[h], %[h], v[122:123]\n\t"
+        "v_add_f64 %[h], %[h], v[124:125]\n\t"
+        "v_add_f64 %[h], %[h], v[126:127]\n\t"
+        "ds_write_b32 %[ca], %[cv]\n\t"
+        "v_add_u32 %[cv], 1, %[cv]\n\t"
+        "L_chain_end_%=:\n\t"
+        "s_waitcnt lgkmcnt(0)\n\t"
+        : [h] "+v"(h), [va] "+v"(va), [n] "+s"(nch), [cv] "+v"(cv)
+        : [ca] "v"(ca)
+        : "memory", "scc",
+          "v64", "v65", "v66", "v67", "v68", "v69", "v70", "v71", "v72", "v73", "v74", "v75", "v76", "v77", "v78", "v79",
+          "v80", "v81", "v82", "v83", "v84", "v85", "v86", "v87", "v88", "v89", "v90", "v91", "v92", "v93", "v94", "v95",
+          "v96", "v97", "v98", "v99", "v100", "v101", "v102", "v103", "v104", "v105", "v106", "v107", "v108", "v109", "v110", "v111",
+          "v112", "v113", "v114", "v115", "v116", "v117", "v118", "v119", "v120", "v121", "v122", "v123", "v124", "v125", "v126", "v127",
+          "v128", "v129", "v130", "v131", "v132", "v133", "v134", "v135", "v136", "v137", "v138", "v139", "v140", "v141", "v142", "v143",
+          "v144", "v145", "v146", "v147", "v148", "v149", "v150", "v151", "v152", "v153", "v154", "v155", "v156", "v157", "v158", "v159",
+          "v160", "v161", "v162", "v163", "v164", "v165", "v166", "v167", "v168", "v169", "v170", "v171", "v172", "v173", "v174", "v175",
+          "v176", "v177", "v178", "v179", "v180", "v181", "v182", "v183", "v184", "v185", "v186", "v187", "v188", "v189", "v190", "v191");
+    return h;
+}
+// Byte address of an LDS object (its offset in the workgroup's LDS).
+__device__ __forceinline__ unsigned lds_offset(const void* p) {
+    return (unsigned)(uintptr_t)(const __attribute__((address_space(3))) char*)p;
+}
 
 template <int R_>
 struct RoleTag { static constexpr int value = R_; };
@@ -481,7 +712,10 @@ __device__ __forceinline__ void tp_body(const PersistArgs& a, const int g) {
     // ---- LDS layout
     double* P = lds;                                   // [NB][PSr]  P[r][k], or chunk slots when W > 0
     double* coef_s = P + (size_t)NB * PSr;             // [PS]       zero past L-1
-    double* stage = coef_s + PS;                       // [2][NB]    V of the last two steps
+    const bool HL = !WIN && a.hl;                      // history rows in LDS
+    const int HR = tp_hist_stride(L);
+    double* hist_s = coef_s + PS;                      // [NB][HR]   when HL
+    double* stage = hist_s + (HL ? (size_t)NB * HR : 0);   // [2][NB]    V of the last two steps
     double* rec_s = stage + 2 * NB;                    // [2][NB]
     double* ext_s = rec_s + 2 * NB;                    // [2][NB]
     double* eval_s = ext_s + 2 * NB;                   // [ne_lds]
@@ -519,6 +753,13 @@ __device__ __forceinline__ void tp_body(const PersistArgs& a, const int g) {
         if (rec_S) rec_S += (size_t)t0 * n;
     }
     for (int k = tid; k < PS; k += kTpThreads) coef_s[k] = k < L ? a.coef[k] : 0.0;
+    if (HL) {   // history rows from the HBM ring (rows past nloc repeat the last neuron)
+        for (int e = tid; e < NB * HR; e += kTpThreads) {
+            const int rr = e / HR, q = e - rr * HR;
+            const int sl = L - 1 - (q < L ? q : (q - L) % L);
+            hist_s[e] = a.ring[(size_t)sl * n + i0 + (rr < nloc ? rr : nloc - 1)];
+        }
+    }
     const int head0 = (int)(st0 % L);
     {
         const int sp = head0 == 0 ? L - 1 : head0 - 1;   // V_{st0-1}
@@ -538,7 +779,7 @@ __device__ __forceinline__ void tp_body(const PersistArgs& a, const int g) {
         for (int e = tid; e < ne + kBitmapBatch; e += kTpThreads) list_s[e] = 0;
         if (csr_lds)
             for (int e = tid; e < ne; e += kTpThreads) eval_s[e] = a.val[E0 + e];
-        if (wave == 4) {   // first edge of each (part, row) scan range
+        if (wave == kTpXchWave) {   // first edge of each (part, row) scan range
             const int rr = lane & (NB - 1), w_end = min(nwp, (lane / NB) * a.cw);
             int pb = 0;
             if (rr < nloc) {
@@ -566,7 +807,7 @@ __device__ __forceinline__ void tp_body(const PersistArgs& a, const int g) {
     const bool win_reg = a.din <= kMaxDinReg;
 #pragma unroll
     for (int j = 0; j < kMaxDinReg; ++j) win[j] = 0.0;
-    if (wave == 4 && lane < nloc) {
+    if (wave == kTpXchWave && lane < nloc) {
         rs = (int)(a.rp[i] - E0);
         re = (int)(a.rp[i + 1] - E0);
         if (win_reg)
@@ -578,8 +819,8 @@ __device__ __forceinline__ void tp_body(const PersistArgs& a, const int g) {
         p[j] = a.np[j];
         asm volatile("" : "+v"(p[j]));
     }
-    const bool producer = wave == 1 || wave == 2 || wave == 3 || wave == 5;
-    const int pw = wave == 5 ? 3 : wave - 1;
+    const bool producer = wave != 0 && wave != kTpXchWave;
+    const int pw = wave - 1 - (wave > kTpXchWave ? 1 : 0);   // product wave index 0..3
     const int part = pw * (64 / NB) + lane / NB;
     int head = head0;
     int lim = st0 < L ? (int)st0 : L - 1;
@@ -588,6 +829,10 @@ __device__ __forceinline__ void tp_body(const PersistArgs& a, const int g) {
         const int nk0 = lim >= 2 ? lim - 1 : 0;
         const int pre0 = W ? min(W, (nk0 + C - 1) / C) : 0;
         if (W && tid < kTpProducers) prod_s[tid] = (unsigned)pre0;
+    }
+    if constexpr (!WIN) {   // lags past the step's last are read by the chain as +0.0
+        for (int e = tid; e < NB * PSr; e += kTpThreads) P[e] = 0.0;
+        __syncthreads();
     }
     if (producer) {
         const int nk = lim >= 2 ? lim - 1 : 0;
@@ -599,8 +844,10 @@ __device__ __forceinline__ void tp_body(const PersistArgs& a, const int g) {
                 const int ka = k < L ? k : L - 1, kb = k + 1 < L ? k + 1 : L - 1;
                 int sa = head - ka; if (sa < 0) sa += L;
                 int sbb = head - kb; if (sbb < 0) sbb += L;
-                *reinterpret_cast<double2*>(P + (size_t)r * PSr + k) =
-                    make_double2(coef_s[k] * a.ring[(size_t)sa * n + ir], coef_s[k + 1] * a.ring[(size_t)sbb * n + ir]);
+                const double pa = coef_s[k] * a.ring[(size_t)sa * n + ir];
+                const double pb = coef_s[k + 1] * a.ring[(size_t)sbb * n + ir];
+                *reinterpret_cast<double2*>(P + (size_t)r * PSr + k) = make_double2(k <= lim ? pa : 0.0,
+                                                                                    k + 1 <= lim ? pb : 0.0);
             }
         }
     }
@@ -611,11 +858,30 @@ __device__ __forceinline__ void tp_body(const PersistArgs& a, const int g) {
     long long st = st0;
     unsigned m = 0;                                    // micro-step index within the launch
     unsigned gb = 0;                                   // chunks of the steps before this one (windowed)
+    unsigned seen = 0;                                 // producers: last chain progress read
     // One loop per wave role: with the roles as branches of a single loop, copies
     // of the producers' in-flight load registers sit on every path and make the
     // chain wave wait for its own global stores (the store counter is shared).
     auto steps = [&](auto role) __attribute__((always_inline)) {
         constexpr int R = decltype(role)::value;
+        // exchange wave, simple synapses, CSR slice in LDS, rows of at most kRegEdges
+        // edges: each lane keeps its row's column codes in registers, so a step's
+        // pull is one batch of spike-word and weight reads, then the in-order adds
+        unsigned pk_r[kRegEdges];
+        double wv_r[kRegEdges];
+        double uin_r[kMaxDinReg];
+        int len_r = 0, len_max = 0;
+        if constexpr (R == 5) {
+            len_r = lane < nloc ? re - rs : 0;
+            int mx = len_r;
+            for (int o = 32; o > 0; o >>= 1) mx = max(mx, __shfl_xor(mx, o));
+            len_max = __builtin_amdgcn_readfirstlane(mx);
+#pragma unroll
+            for (int q = 0; q < kRegEdges; ++q) {
+                pk_r[q] = q < len_r ? eidx_s[rs + q] : 0u;
+                wv_r[q] = q < len_r ? eval_s[rs + q] : 0.0;
+            }
+        }
         for (long long t = 0; t < a.T; ++t) {
             const long long tt = t0 + t;
             const int tb = (int)(t & 1);
@@ -629,10 +895,6 @@ __device__ __forceinline__ void tp_body(const PersistArgs& a, const int g) {
                     double h = 0.0;
                     if (lim >= 1) h = h + c1 * Vprev;
                     double2 bA[C / 2], bB[C / 2];
-                    auto ld = [&](double2* buf, int c) {
-#pragma unroll
-                        for (int j = 0; j < C / 2; ++j) buf[j] = prow[1 + (C / 2) * c + j];
-                    };
                     auto add = [&](const double2* buf, int cnt) {
                         if (cnt == C) {
                             add16_f64_ordered(h, buf);
@@ -646,20 +908,7 @@ __device__ __forceinline__ void tp_body(const PersistArgs& a, const int g) {
                         }
                     };
                     if (!WIN && lane < NB) {   // lanes past NB would only repeat the rows of lanes 0..NB-1
-                        if (nch > 0) ld(bA, 0);
-                        int c = 0;
-                        for (; c + 2 <= nch; c += 2) {
-                            ld(bB, c + 1);
-                            add(bA, C);
-                            lds_counter_store(&cons_s, m * 64u + (unsigned)c + 1u);
-                            if (c + 2 < nch) ld(bA, c + 2);
-                            add(bB, (c + 1 == nch - 1) ? nk - (c + 1) * C : C);
-                            lds_counter_store(&cons_s, m * 64u + (unsigned)c + 2u);
-                        }
-                        if (c < nch) {
-                            add(bA, nk - c * C);
-                            lds_counter_store(&cons_s, m * 64u + (unsigned)c + 1u);
-                        }
+                        if (nch > 0) h = chain_chunks(h, lds_offset(prow + 1), nch, lds_offset(&cons_s), m * 64u + 1u);
                     } else if (WIN && lane < NB) {
                         // windowed: chunk c of this step is chunk gb+c of the stream, in slot
                         // (gb+c) mod W; it is read once every producer wave has written it, and
@@ -714,7 +963,8 @@ __device__ __forceinline__ void tp_body(const PersistArgs& a, const int g) {
                             release(gb + (unsigned)c + 1u);
                         }
                     }
-                    const double leak = -(Vprev - p[2]) / p[3];   // before the barrier: the wait hides it
+                    double leak = -(Vprev - p[2]) / p[3];   // before the barrier: the wait hides it
+                    asm volatile("" : "+v"(leak));          // (kept here, not sunk past the barrier)
                     __syncthreads();   // rec / ext of this step and the products of the next are ready
                     if (__builtin_amdgcn_readfirstlane(abort_s)) break;
                     if (own) {                                  // flif_gl.c:50-100
@@ -732,6 +982,11 @@ __device__ __forceinline__ void tp_body(const PersistArgs& a, const int g) {
                     if (own) {
                         a.ring[(size_t)head * n + i] = V;
                         stage[sb * NB + lane] = V;
+                        if (HL) {
+                            const int e = L - 1 - head;
+                            hist_s[lane * HR + e] = V;
+                            for (int q = e; q < HR - L; q += L) hist_s[lane * HR + L + q] = V;
+                        }
                         tp = tp + a.dt;
                         if (mi == a.micro - 1) {
                             if (rec_V) rec_V[(size_t)t * n + i] = V;
@@ -791,25 +1046,95 @@ __device__ __forceinline__ void tp_body(const PersistArgs& a, const int g) {
                     }
                     __syncthreads();
                     if (__builtin_amdgcn_readfirstlane(abort_s)) break;
+                } else if constexpr (R == 5) {
+                    // ---------------- exchange with the row pull from registers (simple
+                    // synapses, CSR slice in LDS, rows of at most kRegEdges edges): one batch
+                    // of spike-word and weight reads, then the in-order adds (sparse.c:68-76;
+                    // a silent edge adds +0.0, which leaves the sum unchanged).  The inputs
+                    // of macro step t (uin_r) were loaded during step t-1.
+                    if (t == 0 && mi == 0 && win_reg) {
+#pragma unroll
+                        for (int j = 0; j < kMaxDinReg; ++j) uin_r[j] = (j < a.din) ? a.u[tt * a.din + j] : 0.0;
+                    }
+                    const bool aborted = !gather_spikes(a.xch, nw, NB, wpg, n, st, st0, S_in, bits_s,
+                                                        a.abort_flag, &abort_s, lane);
+                    __builtin_amdgcn_wave_barrier();   // bits_s written by this wave, read by its lanes
+                    if (!aborted) {
+                        unsigned bw[kRegEdges];
+#pragma unroll
+                        for (int q = 0; q < kRegEdges; ++q)
+                            if (q < len_max) bw[q] = bits_s[pk_r[q] >> 5];
+                        double sum = 0.0;
+#pragma unroll
+                        for (int q = 0; q < kRegEdges; ++q)
+                            if (q < len_max) {
+                                const bool hit = q < len_r && ((bw[q] >> (pk_r[q] & 31)) & 1u);
+                                sum = sum + (hit ? wv_r[q] : 0.0);
+                            }
+                        if (lane < nloc) {
+                            rec_s[sb * NB + lane] = sum;
+                            if (mi == 0) {
+                                double su = 0.0;
+                                if (win_reg) {
+#pragma unroll
+                                    for (int j = 0; j < kMaxDinReg; ++j)
+                                        if (j < a.din) su = su + win[j] * uin_r[j];
+                                } else {
+                                    for (int j = 0; j < a.din; ++j)
+                                        su = su + a.W_in[(size_t)i * a.din + j] * a.u[tt * a.din + j];
+                                }
+                                ext_s[tb * NB + lane] = a.gain * su;
+                            }
+                        }
+                    }
+                    if (mi == 0 && win_reg && t + 1 < a.T) {   // the next macro step's inputs
+#pragma unroll
+                        for (int j = 0; j < kMaxDinReg; ++j) uin_r[j] = (j < a.din) ? a.u[(tt + 1) * a.din + j] : 0.0;
+                    }
+                    __syncthreads();
+                    if (__builtin_amdgcn_readfirstlane(abort_s)) break;
                 } else if constexpr (!WIN) {
                     // ---------------- products of step st+1, behind the chain of step st
                     const int headn = head + 1 == L ? 0 : head + 1;
                     const int limn = lim < L - 1 ? lim + 1 : L - 1;
                     const int nkn = limn >= 2 ? limn - 1 : 0;
                     const int nchn = (nkn + C - 1) / C;
-                    // V_{st+1-k}, k >= 3, is final since the last barrier: loads run three chunks
-                    // ahead of the stores.  The slot of lag k = kbase + off is
-                    // (headn - kbase - off) mod L with off < 2L; lag 2 (replaced by the staged
-                    // value) and lags past lim are read but discarded.
-                    const int sbase = headn - (2 + part * PER);
-                    const double* ring_r = a.ring + ir;
-                    auto ldr = [&](double* v, int c) {
+                    // V_{st+1-k}, k >= 3, is final since the last barrier: loads run D - 1
+                    // chunks ahead of the stores.  Lag k of step st+1 sits in slot
+                    // (headn - k) mod L; the lane's two lags of chunk c are
+                    // k = 2 + c*C + part*PER + j.  Each lane keeps the byte offset of its next
+                    // slot per j (32-bit: N*L*8 < 2^31 for this kernel) and steps it back C
+                    // slots per chunk, in chunk order; chunks past the first exist only when
+                    // L > C, so one wrap suffices.  Lag 2 (replaced by the staged value) and
+                    // lags past lim are read but discarded.
+                    const int rowb = n * 8;
+                    unsigned offb[PER];
+#pragma unroll
+                    for (int j = 0; j < PER; ++j) {
+                        int sl = (headn - (2 + part * PER + j)) % L;
+                        sl += sl < 0 ? L : 0;
+                        offb[j] = (unsigned)(sl * rowb + ir * 8);
+                    }
+                    const int stepb = C * rowb, wrapb = L * rowb;
+                    const char* ring_b = reinterpret_cast<const char*>(a.ring);
+                    // history rows in LDS: lag k0 = 2 + c*C + part*PER of chunk c is entry
+                    // (L - 1 - headn + k0) mod L of row r, its next PER - 1 lags the entries after it
+                    int hA = (L - 1 - headn + 2 + part * PER) % L;
+                    const double* hrow = hist_s + (size_t)r * HR;
+                    auto ldr = [&](double* v, int) {   // called for chunks 0, 1, 2, ... in order
+                        if (HL) {
+#pragma unroll
+                            for (int j = 0; j < PER; ++j) v[j] = hrow[hA + j];
+                            hA += C;
+                            hA -= hA >= L ? L : 0;
+                            return;
+                        }
 #pragma unroll
                         for (int j = 0; j < PER; ++j) {
-                            int s = sbase - (c * C + j);
-                            s += s < 0 ? L : 0;
-                            s += s < 0 ? L : 0;
-                            v[j] = ring_r[(size_t)s * n];
+                            v[j] = *reinterpret_cast<const double*>(ring_b + offb[j]);
+                            int o = (int)offb[j] - stepb;
+                            o += o < 0 ? wrapb : 0;
+                            offb[j] = (unsigned)o;
                         }
                     };
                     bool stalled = false;
@@ -821,16 +1146,21 @@ __device__ __forceinline__ void tp_body(const PersistArgs& a, const int g) {
                             pr[j] = make_double2(coef_s[k] * v[2 * j], coef_s[k + 1] * v[2 * j + 1]);
                         }
                         // the chain of step st is past chunk c (or does not read it) and its tail
-                        // of step st-1 has staged V_{st-1}
+                        // of step st-1 has staged V_{st-1} (the last value read is kept: the
+                        // chain is often chunks ahead)
                         const unsigned target = m * 64u + (unsigned)(c + 1 < nch ? c + 1 : nch);
-                        long long spins = 0;
-                        while ((int)(lds_counter_load(&cons_s) - target) < 0 && !stalled) {
-                            if (++spins > kTpSpinLimit) {
-                                stalled = true;
-                                __hip_atomic_store(&abort_s, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-                                __hip_atomic_store(a.abort_flag, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        if ((int)(seen - target) < 0) {
+                            long long spins = 0;
+                            while (!stalled) {
+                                seen = lds_counter_load(&cons_s);
+                                if ((int)(seen - target) >= 0) break;
+                                if (++spins > kTpSpinLimit) {
+                                    stalled = true;
+                                    __hip_atomic_store(&abort_s, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                                    __hip_atomic_store(a.abort_flag, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                                }
+                                __builtin_amdgcn_s_sleep(1);
                             }
-                            __builtin_amdgcn_s_sleep(1);
                         }
                         if (c == 0 && part == 0)
                             pr[0].x = coef_s[2] * stage[(int)((st - 1) & 1) * NB + r];   // lag 2: V_{st-1}
@@ -838,24 +1168,30 @@ __device__ __forceinline__ void tp_body(const PersistArgs& a, const int g) {
 #pragma unroll
                         for (int j = 0; j < PER / 2; ++j) dst[j] = pr[j];
                     };
-                    double vA[PER], vB[PER], vC[PER], vD[PER];
-                    if (nchn > 0) ldr(vA, 0);
-                    if (nchn > 1) ldr(vB, 1);
-                    if (nchn > 2) ldr(vC, 2);
+                    // chunk q is loaded into vb[q % D], D - 1 chunks ahead of its products
+                    constexpr int D = kTpLoadDepth;
+                    double vb[D][PER];
+#pragma unroll
+                    for (int j = 0; j < D - 1; ++j)
+                        if (j < nchn) ldr(vb[j], j);
                     int c = 0;
-                    for (; c + 4 <= nchn; c += 4) {
-                        ldr(vD, c + 3);
-                        put(vA, c);
-                        if (c + 4 < nchn) ldr(vA, c + 4);
-                        put(vB, c + 1);
-                        if (c + 5 < nchn) ldr(vB, c + 5);
-                        put(vC, c + 2);
-                        if (c + 6 < nchn) ldr(vC, c + 6);
-                        put(vD, c + 3);
+                    for (; c + D <= nchn; c += D) {
+#pragma unroll
+                        for (int j = 0; j < D; ++j) {
+                            if (c + j + D - 1 < nchn) ldr(vb[(j + D - 1) % D], c + j + D - 1);
+                            put(vb[j], c + j);
+                        }
                     }
-                    if (c < nchn) put(vA, c);
-                    if (c + 1 < nchn) put(vB, c + 1);
-                    if (c + 2 < nchn) put(vC, c + 2);
+#pragma unroll
+                    for (int j = 0; j < D - 1; ++j)
+                        if (c + j < nchn) put(vb[j], c + j);
+                    if (nchn > 0) {   // the chain reads the lags past limn (last chunk only) as +0.0
+#pragma unroll
+                        for (int j = 0; j < PER; ++j) {
+                            const int k = 2 + (nchn - 1) * C + part * PER + j;
+                            if (k > limn) P[(size_t)r * PS + k] = 0.0;
+                        }
+                    }
                     __syncthreads();
                     if (__builtin_amdgcn_readfirstlane(abort_s)) break;
                 } else {
@@ -950,7 +1286,13 @@ __device__ __forceinline__ void tp_body(const PersistArgs& a, const int g) {
         }
     };
     if (wave == 0) steps(RoleTag<0>{});
-    else if (wave == 4) steps(RoleTag<4>{});
+    else if (wave == kTpXchWave) {
+        const int len = lane < nloc ? re - rs : 0;
+        if (NB == 16 && !bitmap && csr_lds && a.stype == ST_SIMPLE && __all(len <= kRegEdges))
+            steps(RoleTag<5>{});
+        else
+            steps(RoleTag<4>{});
+    }
     else steps(RoleTag<1>{});
     if (own) {
         a.V[i] = V;
@@ -989,6 +1331,7 @@ struct Plan {
     int tp = 0;                 // FLIF-GL with the trailing product stage (k_persist_tp)
     int rpm = 0, cw = 0;        // row pull over spike-word bitmaps (k_persist_tp only)
     int win = 0;                // k_persist_tp product window in chunks (0: whole rows)
+    int hl = 0;                 // k_persist_tp: history rows in LDS (products read no HBM)
     size_t lds = 0;
 };
 
@@ -1028,6 +1371,11 @@ bool persist_plan(spires_dev* g, int ncu, Plan* pl, bool batch = false) {
     // Whole product rows when they fit in LDS; otherwise a window of W 32-lag
     // chunks per row (the largest of 16, 8, 4 that fits), refilled by the
     // product waves while the chain runs.
+    // whole-row product plans keep the workgroup's history rows in LDS when they fit
+    auto add_history = [&](int nb) {
+        const size_t hb = (size_t)nb * tp_hist_stride(L) * 8;
+        if (pl->win == 0 && pl->lds + hb <= cap) { pl->hl = 1; pl->lds += hb; }
+    };
     auto fill_tp = [&](int nb, bool fast_only) -> bool {
         const int G = (n + nb - 1) / nb;
         const int wpg = (nb + 31) / 32;
@@ -1046,10 +1394,12 @@ bool persist_plan(spires_dev* g, int ncu, Plan* pl, bool batch = false) {
             if (!win) return false;
         } else if (nb == 64) {
             return false;   // whole rows at 64 neurons per workgroup: not instantiated (32 fits first)
+        } else if ((long long)n * L * 8 >= (1ll << 31)) {
+            return false;   // whole-row product waves address the history ring with 32-bit offsets
         }
         const int64_t me = max_edges(nb);
         const int nwp = (n + 31) / 32;
-        pl->nb = nb; pl->G = G; pl->ring_lds = 0; pl->tp = 1; pl->rpm = 0; pl->cw = 0; pl->win = win;
+        pl->nb = nb; pl->G = G; pl->ring_lds = 0; pl->tp = 1; pl->rpm = 0; pl->cw = 0; pl->win = win; pl->hl = 0;
         const bool dense = nb <= 32 && g->stype == ST_SIMPLE && me + kBitmapBatch < 65536 &&
                            2 * me > (int64_t)nb * nwp;
         if (dense) {
@@ -1063,6 +1413,7 @@ bool persist_plan(spires_dev* g, int ncu, Plan* pl, bool batch = false) {
                 pl->cw = ((nwp + parts - 1) / parts + 7) / 8 * 8;
                 pl->ne_lds = wl ? (int)std::max<int64_t>(me, 1) : 0;
                 pl->lds = base + bm + (size_t)pl->ne_lds * 8;
+                add_history(nb);
                 return true;
             }
         }
@@ -1070,6 +1421,7 @@ bool persist_plan(spires_dev* g, int ncu, Plan* pl, bool batch = false) {
         if (fast_only && !el) return false;
         pl->ne_lds = el ? (int)std::max<int64_t>(me, 1) : 0;
         pl->lds = base + (size_t)pl->ne_lds * 12;
+        add_history(nb);
         return true;
     };
     if (gl) {
@@ -1131,11 +1483,11 @@ namespace {
 int prepare_launch(spires_dev* g, const Plan& pl) {
     const int wpg = (pl.nb + 31) / 32;
     const size_t nw = (size_t)pl.G * wpg;
-    if (g->xch_words < 2 * nw) {
+    if (g->xch_words < 2 * nw * kXchStride) {
         dfree(g->xch);
-        int rc = dalloc(&g->xch, 2 * nw);
+        int rc = dalloc(&g->xch, 2 * nw * kXchStride);
         if (rc) return rc;
-        g->xch_words = 2 * nw;
+        g->xch_words = 2 * nw * kXchStride;
         g->xch_plan_G = 0;
     }
     if (g->xch_plan_G != pl.G) {
@@ -1173,7 +1525,7 @@ PersistArgs make_args(spires_dev* g, const Plan& pl, const double* u_series, lon
     memset(&a, 0, sizeof(a));
     a.n = g->n; a.din = g->din; a.L = g->L; a.G = pl.G; a.wpg = (pl.nb + 31) / 32; a.micro = g->micro;
     a.stype = g->stype; a.ne_lds = pl.ne_lds;
-    a.rpm = pl.rpm; a.cw = pl.cw; a.rbm = g->rbm; a.win = pl.win;
+    a.rpm = pl.rpm; a.cw = pl.cw; a.rbm = g->rbm; a.win = pl.win; a.hl = pl.hl;
     a.st0 = g->step; a.t0 = t0; a.T = T;
     a.gain = g->d.gain; a.dt = g->d.dt;
     memcpy(a.np, g->d.np, sizeof(a.np));
@@ -1377,11 +1729,12 @@ extern "C" int spires_dev_exchange_floor(int G, long long T, double* us_per_step
         return rc;
     };
     int rc = 0;
-    if ((rc = dalloc(&xch, 2 * (size_t)G)) || (rc = dalloc(&S, (size_t)G * 32)) || (rc = dalloc(&ab, 1))) return done(rc);
+    if ((rc = dalloc(&xch, 2 * (size_t)G * kXchStride)) || (rc = dalloc(&S, (size_t)G * 32)) || (rc = dalloc(&ab, 1)))
+        return done(rc);
     hipError_t e = hipStreamCreate(&st);
     if (e == hipSuccess) e = hipEventCreate(&e0);
     if (e == hipSuccess) e = hipEventCreate(&e1);
-    if (e == hipSuccess) e = hipMemsetAsync(xch, 0, 2 * (size_t)G * 8, st);
+    if (e == hipSuccess) e = hipMemsetAsync(xch, 0, 2 * (size_t)G * kXchStride * 8, st);
     if (e == hipSuccess) e = hipMemsetAsync(S, 0, (size_t)G * 32, st);
     if (e == hipSuccess) e = hipMemsetAsync(ab, 0, 4, st);
     void* args[] = {&xch, &G, &T, &S, &ab};
