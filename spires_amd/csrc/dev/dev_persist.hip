@@ -891,7 +891,9 @@ __device__ __forceinline__ void tp_body(const PersistArgs& a, const int g) {
                 const int nch = (nk + C - 1) / C;
                 if constexpr (R == 0) {
                     // ---------------- chain: H = c_1 V_{st-1} + sum_k P[k]
-                    const double Vprev = stage[(int)((st - 1) & 1) * NB + r];
+                    // V_{st-1}: this lane's own V after the first step of the launch (the tail
+                    // stored the same value in the ring and the stage), the staged ring value before
+                    const double Vprev = m == 0u ? stage[(int)((st - 1) & 1) * NB + r] : V;
                     double h = 0.0;
                     if (lim >= 1) h = h + c1 * Vprev;
                     double2 bA[C / 2], bB[C / 2];
