@@ -254,12 +254,13 @@ def run_batched(args):
     """64 FLIF-GL reservoirs of uneven N (log-U[256, 8192]), L, alpha
     and topology (ER p=0.05 / BA m=5), N(0,1) weights rescaled to rho, on one
     13-channel input of AR(1) class sequences, advanced in time blocks of K
-    steps.  The reservoirs that fit the product-wave kernel run as one
-    TimeBlock: their launches (packed per neurons-per-workgroup class, at most
-    one workgroup per CU) are captured once in a HIP graph and replayed every
-    block; the others run one launch each per block.  K is picked by timing
-    one block per candidate; the same blocks are also timed with eager
-    launches for comparison."""
+    steps captured once in a HIP graph and replayed every block.  By default
+    all 64 reservoirs run in one cooperative launch of 64-neuron slices per
+    block (TB_ONE_LAUNCH); with --packed the reservoirs that fit the
+    product-wave kernel run as launches packed per neurons-per-workgroup class
+    (at most one workgroup per CU) and the others one launch each per block.
+    K is picked by timing one block per candidate; the product-wave launches
+    are also timed eagerly for comparison."""
     import numpy as np
     import spires_amd as sa
     from spires_amd import workloads
@@ -286,7 +287,8 @@ def run_batched(args):
     flags = sa.REC_STATE | sa.REC_APPEND
     # the graph's reservoirs and flags: the product-wave launches, or (one launch)
     # every reservoir as 64-neuron one-wave slices of a single launch
-    gsel, gflags, gsingles = (list(range(len(res))), sa.REC_STATE | sa.TB_ONE_LAUNCH, []) if args.one_launch \
+    one_launch = not args.packed
+    gsel, gflags, gsingles = (list(range(len(res))), sa.REC_STATE | sa.TB_ONE_LAUNCH, []) if one_launch \
         else (tp, sa.REC_STATE, singles)
 
     def eager_block(t, K):
@@ -362,7 +364,7 @@ def run_batched(args):
                                "13-channel AR(1) class sequences (512 sequences, %d frames)" % (total_n, T),
                    "block_k": K,
                    "graph": ("HIP graph per time block: one launch of 64-neuron slices (all %d reservoirs)" % len(gsel))
-                   if args.one_launch else "HIP graph per time block (%d product-wave reservoirs)" % len(tp)},
+                   if one_launch else "HIP graph per time block (%d product-wave reservoirs)" % len(tp)},
         "k_autotune_us_per_step": ktimes,
         "eager_us_per_step": eager_us,
         "launches_per_block": launches,
@@ -370,7 +372,7 @@ def run_batched(args):
         "ms_per_step": wall / steps * 1e3,
         "roofline": {"bound": "hbm", "achieved": alg, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": alg / HBM_PEAK_GBS,
                      "traffic": None,
-                     "kernel": "k_mux_gl (one launch per block)" if args.one_launch
+                     "kernel": "k_mux_gl (one launch per block)" if one_launch
                      else "k_persist_tp_batch (graph) + k_persist (all launches of a block)"},
     }
     print(json.dumps(line), flush=True)
@@ -773,8 +775,9 @@ def main():
                     help="time one batched ridge fit of 16 N=2048 search trials at the 6000-step rung")
     ap.add_argument("--batched", action="store_true",
                     help="run the batched workload (64 uneven reservoirs) instead of the step benchmark")
-    ap.add_argument("--one-launch", action="store_true",
-                    help="with --batched: every reservoir in one launch of 64-neuron slices per time block")
+    ap.add_argument("--packed", action="store_true",
+                    help="with --batched: product-wave launches packed per neurons-per-workgroup class "
+                         "instead of one launch of 64-neuron slices per time block")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the library runs one reservoir per device and has no "

@@ -1849,19 +1849,22 @@ __device__ __forceinline__ gptr<T> to_global(T* p) {
     return (gptr<T>)p;
 }
 
-// gl_seg_hbm with each slot's base a wave-uniform pointer and the lane's
-// neuron a 32-bit byte offset: the loads take a scalar base and one offset
-// register instead of a 64-bit address per lane and load.
+// h += sum_{k=1..cnt} c[k] * ring[((head-k) mod L)*n + i], ascending k, one
+// multiply and one add per lag (flif_gl.c:66-72).  The ring is read through a
+// buffer resource: the lane's neuron is a fixed byte offset (voff), the slot a
+// scalar offset computed per load, so a load costs no address registers;
+// blocks of 16 lags, NBLK blocks in flight.  The coefficients come from LDS
+// (cs, broadcast reads).
 template <int NBLK>
-__device__ __forceinline__ double gl_seg_mux(double h, gptr<const double> ring, unsigned off,
-                                             gptr<const double> c, int kb, int sb, int cnt, size_t stride) {
+__device__ __forceinline__ double gl_seg_mux(double h, __amdgpu_buffer_rsrc_t rs, unsigned voff, const double* cs,
+                                             int head, int cnt, int L, unsigned rowb) {
     constexpr int B = 16;
+    const int kb = 1, sb = head - 1;
     const int nblk = cnt / B;
     double v[NBLK][B];
     auto at = [&](int slot) {
-        const int sl = __builtin_amdgcn_readfirstlane(slot);   // uniform: a scalar base per slot
-        gptr<const char> base = (gptr<const char>)(ring + (size_t)sl * stride);
-        return *(gptr<const double>)(base + off);
+        slot += slot < 0 ? L : 0;
+        return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rs, voff, (unsigned)slot * rowb, 0));
     };
     auto load = [&](double* buf, int blk) {
 #pragma unroll
@@ -1876,13 +1879,13 @@ __device__ __forceinline__ double gl_seg_mux(double h, gptr<const double> ring, 
             const int blk = b0 + u;
             if (blk < nblk) {
 #pragma unroll
-                for (int j = 0; j < B; ++j) h = h + c[kb + blk * B + j] * v[u][j];
+                for (int j = 0; j < B; ++j) h = h + cs[kb + blk * B + j] * v[u][j];
                 if (blk + NBLK < nblk) load(v[u], blk + NBLK);
             }
         }
     }
     const int q = nblk * B, rem = cnt - q;
-    for (int j = 0; j < rem; ++j) h = h + c[kb + q + j] * at(sb - q - j);
+    for (int j = 0; j < rem; ++j) h = h + cs[kb + q + j] * at(sb - q - j);
     return h;
 }
 
@@ -1983,6 +1986,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) k_
     const PersistArgs* __restrict__ table, const int2* __restrict__ wg_map) {
     __shared__ unsigned bits_s[kMuxMaxWords];
     __shared__ int abort_s;
+    extern __shared__ __align__(16) double coef_s[];   // [L] GL coefficients of the slice's reservoir
     const int2 mw = wg_map[blockIdx.x];
     const PersistArgs& a = table[mw.x];
     const int lane = threadIdx.x, g = mw.y;
@@ -2031,6 +2035,10 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) k_
         asm volatile("" : "+v"(p[j]));
     }
     const unsigned ring_off = (unsigned)i * 8u;   // byte offset of the lane's neuron in a ring slot
+    const unsigned rowb = (unsigned)n * 8u;        // bytes per ring slot (N*L*8 < 2^31: checked on the host)
+    const __amdgpu_buffer_rsrc_t ring_rs =
+        __builtin_amdgcn_make_buffer_rsrc((void*)a.ring, (short)0, (int)((unsigned)L * rowb), 0x00020000);
+    for (int k = lane; k < L; k += 64) coef_s[k] = coef[k];
     __syncthreads();
     long long st = st0;
     int head = (int)(st % L);
@@ -2040,10 +2048,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) k_
         double ext = 0.0;
         for (int mi = 0; mi < micro; ++mi, ++st) {
             // GL history of this micro-step (lags 1..lim in ascending order)
-            const int n1 = lim < head ? lim : head;
-            double H = 0.0;
-            H = gl_seg_mux<kMuxBlocks>(H, ring_c, ring_off, coef, 1, head - 1, n1, (size_t)n);
-            H = gl_seg_mux<kMuxBlocks>(H, ring_c, ring_off, coef, n1 + 1, head - n1 - 1 + L, lim - n1, (size_t)n);
+            const double H = gl_seg_mux<kMuxBlocks>(0.0, ring_rs, ring_off, coef_s, head, lim, L, rowb);
             // spikes of micro-step st-1, CSR rows, input drive
             const bool aborted = !mux_gather(xch, nw, n, st, st0, S_in, bits_s, abort_flag, &abort_s, lane);
             __syncthreads();
@@ -2058,8 +2063,10 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) k_
             const double x = ext + rec;
             spk = false;
             if (tp > p[6]) {
+                // V_{st-1}: the lane's own V after the launch's first step (the same value
+                // went to the ring), the ring value before
                 const int sp = head == 0 ? L - 1 : head - 1;
-                const double Vp = ring[(size_t)sp * n + i];
+                const double Vp = st == st0 ? ring[(size_t)sp * n + i] : V;
                 const double rhs = (-(Vp - p[2]) / p[3]) + x + p[5];
                 V = p[4] * rhs - H;
                 if (V >= p[0]) { V = p[1]; spk = true; tp = 0.0; }
@@ -2139,6 +2146,7 @@ extern "C" int spires_dev_timeblock_create(spires_dev* const* gs, int count, lon
         return SPIRES_DEV_EINVAL;
     }
     const bool one = (flags & SPIRES_DEV_TB_ONE_LAUNCH) != 0;
+    size_t mux_lds = 0;   // one-launch kernel: the GL coefficients of the longest history
     const int ncu = num_cus();
     std::vector<Plan> plans(count);
     for (int r = 0; r < count; ++r) {
@@ -2170,7 +2178,16 @@ extern "C" int spires_dev_timeblock_create(spires_dev* const* gs, int count, lon
             total += (gs[r]->n + kMuxNb - 1) / kMuxNb;
         }
         int per_cu = 0;
-        SDEV_CK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k_mux_gl, 64, 0));
+        int Lmax = 0;
+        for (int r = 0; r < count; ++r) {
+            Lmax = std::max(Lmax, gs[r]->L);
+            if ((long long)gs[r]->n * gs[r]->L * 8 >= (1ll << 31)) {
+                set_error("one-launch time blocks address a history ring of at most 2^31 bytes");
+                return SPIRES_DEV_EINVAL;
+            }
+        }
+        mux_lds = (size_t)Lmax * 8;
+        SDEV_CK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k_mux_gl, 64, mux_lds));
         if ((long long)per_cu * ncu < total) {
             set_error("one-launch time block: " + std::to_string(total) + " slices do not fit the " +
                       std::to_string((long long)per_cu * ncu) + " resident workgroups");
@@ -2285,7 +2302,7 @@ extern "C" int spires_dev_timeblock_create(spires_dev* const* gs, int count, lon
         const int2* mp = tb->map + map_off[q];
         void* kargs[] = {(void*)&tab, (void*)&mp};
         if (nbs[q] == kMuxNb)
-            e = hipLaunchCooperativeKernel((const void*)k_mux_gl, dim3(G), dim3(64), kargs, 0, tb->stream);
+            e = hipLaunchCooperativeKernel((const void*)k_mux_gl, dim3(G), dim3(64), kargs, mux_lds, tb->stream);
         else
             e = hipLaunchCooperativeKernel(nbs[q] == 32 ? (const void*)k_persist_tp_batch<32>
                                                         : (const void*)k_persist_tp_batch<16>,
