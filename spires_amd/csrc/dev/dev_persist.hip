@@ -2282,6 +2282,15 @@ extern "C" int spires_dev_timeblock_create(spires_dev* const* gs, int count, lon
         nbs.push_back(nb);
         if (nb != kMuxNb) (nb == 32 ? lds32 : lds16) = std::max(nb == 32 ? lds32 : lds16, l);
     }
+    // one launch: the slices of the shortest histories first (then the smallest reservoirs).
+    // Workgroups are dispatched in order, so every CU gets a like mix either way; with the
+    // short histories first (the older waves) config 2's block is 6 % shorter than with the
+    // longest first or in the caller's order (187-190 against 200 us per step).
+    if (one)
+        std::stable_sort(map.begin(), map.end(), [&](const int2& x, const int2& y) {
+            if (tb->gs[x.x]->L != tb->gs[y.x]->L) return tb->gs[x.x]->L < tb->gs[y.x]->L;
+            return tb->gs[x.x]->n < tb->gs[y.x]->n;
+        });
     if ((rc = dalloc(&tb->table, table.size())) || (rc = dalloc(&tb->map, map.size())) ||
         (rc = dalloc(&tb->items, items.size())))
         return fail(rc);
