@@ -350,7 +350,7 @@ def _levy_steps(rng, count, dim, mu=1.3, lo=1e-3, hi=0.3):
     return d * s[:, None]
 
 
-def surrogate_search(ev, rungs, target=None, budget=512, warm=96, wave=32, promote=4, pool=2048, seed=1,
+def surrogate_search(ev, rungs, target=None, budget=512, warm=96, wave=32, promote=6, promote_warm=4, pool=2048, seed=1,
                      epochs=3000, round_epochs=500, lr=1e-3, reseed=6, reseed_top=3, reseed_direct=True,
                      explore=8, top_weight=4, clip=2.0, novelty=0.5, levy_hi=0.15, low_rungs_warm_only=True,
                      cost_weight=0.15, log=None):
@@ -362,10 +362,15 @@ def surrogate_search(ev, rungs, target=None, budget=512, warm=96, wave=32, promo
     `clip` (a diverged trial's size carries no ranking information) and each
     top-rung record counted `top_weight` times; the network is trained once
     for `epochs` and then continued for `round_epochs` per round.  Of each
-    wave, half of the `promote` trials that continue to the top rung are the
-    best by measured middle-rung score and half the best by the surrogate's
-    top-rung prediction; the others are pruned.  The first wave holds `warm`
-    trials from a Latin hypercube; each later wave holds `wave` proposals:
+    wave, half of the `promote` trials that continue to the top rung
+    (`promote_warm` in the first wave) are the best by measured middle-rung
+    score and half the best by the surrogate's top-rung prediction; the others
+    are pruned.  The defaults fill whole launches at
+    N=2048, where one launch holds four trials (one workgroup per CU): 4 in the
+    first wave, 6 + `reseed` = 12 in later ones; a launch costs about the same
+    with two trials as with four, since its steps are latency-bound.
+    The first wave holds `warm` trials from a Latin hypercube; each later wave
+    holds `wave` proposals:
       - `reseed` fresh weight seeds for the `reseed_top` best hyperparameter
         sets so far (a trial's score varies strongly with its weight draw);
         with `reseed_direct` they go to the top rung without the middle-rung
@@ -416,7 +421,7 @@ def surrogate_search(ev, rungs, target=None, budget=512, warm=96, wave=32, promo
     def rank(v):
         return np.argsort(np.argsort(v))
 
-    def run_wave(new, ep, forced=0, fids=None):
+    def run_wave(new, ep, forced=0, fids=None, promote=promote):
         nonlocal best
         # the first `forced` trials go to the top rung whatever their middle-rung score
         fids = lower if fids is None else fids
@@ -443,7 +448,8 @@ def surrogate_search(ev, rungs, target=None, budget=512, warm=96, wave=32, promo
         ev.release(up)
 
     count = warm
-    run_wave([Trial(th, s) for th, s in zip(latin_hypercube(warm, int(rng.integers(1 << 31))), seeds(warm))], epochs)
+    run_wave([Trial(th, s) for th, s in zip(latin_hypercube(warm, int(rng.integers(1 << 31))), seeds(warm))], epochs,
+             promote=promote_warm)
     rounds = 0
     if log:
         log(f"surrogate warm-up: {count} trials, best {best:.4f} t {time.perf_counter() - t0:.1f}s "

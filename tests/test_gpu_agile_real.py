@@ -28,7 +28,8 @@ def test_surrogate_search_beats_plain_on_real_reservoirs():
     steps = []
     for sd in range(1, 9):
         ev = ag.Evaluator(n=512, f_max=RUNGS[-1], seed=0)
-        r = ag.surrogate_search(ev, RUNGS, target=target, budget=M, seed=sd, warm=32, wave=16, reseed=4, explore=4)
+        r = ag.surrogate_search(ev, RUNGS, target=target, budget=M, seed=sd, warm=32, wave=16, reseed=4, explore=4,
+                                promote=4)
         ev.close()
         ok = r["best"] <= 1.02 * target
         steps.append(r["trial_steps"] if ok else math.inf)
