@@ -453,6 +453,7 @@ constexpr int kTpProducers = 4;
 // than a product wave, whose last chunks would then finish after the chain.
 constexpr int kTpXchWave = 4;
 constexpr int kTpLoadDepth = 8;      // ring-load buffers of a product wave (chunks in flight + 1)
+constexpr int kTpProdUnit = 2;       // chunks a product wave writes per poll of the chain's progress
 constexpr int kRegEdges = 12;        // row pull from registers (16 neurons per workgroup): rows of at most 12 edges
 constexpr long long kTpSpinLimit = 1ll << 24;
 constexpr int kBitmapBatch = 16;     // weights loaded per batch of the bitmap pull from HBM (half from LDS)
@@ -1143,53 +1144,72 @@ __device__ __forceinline__ void tp_body(const PersistArgs& a, const int g) {
                         }
                     };
                     bool stalled = false;
-                    auto put = [&](const double* v, int c) {
-                        double2 pr[PER / 2];
+                    // chunk q is loaded into vb[q % D], D - 1 chunks ahead of its products
+                    constexpr int D = kTpLoadDepth;
+                    constexpr int U = kTpProdUnit;
+                    static_assert(D % U == 0, "units of chunks tile the load ring");
+                    double vb[D][PER];
+                    // Units of U chunks (cnt <= U in the last one): the coefficient pairs of the
+                    // unit and, when the last value read is not enough, the chain's progress are
+                    // read together, so a unit costs one LDS round trip (a wave's LDS reads
+                    // complete in order, so any wait also waits for the loads in flight).  The
+                    // unit's products are written once the chain of step st is past its last
+                    // chunk (or does not read it) and the tail of step st-1 has staged V_{st-1};
+                    // then the history of the chunks D - 1 ahead is loaded into the freed slots.
+                    auto unit = [&](int c0, int s0, int cnt) {
+                        const int cl = c0 + cnt - 1;
+                        const unsigned target = m * 64u + (unsigned)(cl + 1 < nch ? cl + 1 : nch);
+                        const bool need = (int)(seen - target) < 0;
+                        unsigned fresh = seen;
+                        if (need) fresh = lds_counter_load(&cons_s);
+                        double2 pr[U][PER / 2];
 #pragma unroll
-                        for (int j = 0; j < PER / 2; ++j) {
-                            const int k = 2 + c * C + part * PER + 2 * j;
-                            pr[j] = make_double2(coef_s[k] * v[2 * j], coef_s[k + 1] * v[2 * j + 1]);
-                        }
-                        // the chain of step st is past chunk c (or does not read it) and its tail
-                        // of step st-1 has staged V_{st-1} (the last value read is kept: the
-                        // chain is often chunks ahead)
-                        const unsigned target = m * 64u + (unsigned)(c + 1 < nch ? c + 1 : nch);
-                        if ((int)(seen - target) < 0) {
+                        for (int u = 0; u < U; ++u)
+                            if (u < cnt) {
+#pragma unroll
+                                for (int j = 0; j < PER / 2; ++j) {
+                                    const int k = 2 + (c0 + u) * C + part * PER + 2 * j;
+                                    pr[u][j] = make_double2(coef_s[k] * vb[s0 + u][2 * j],
+                                                            coef_s[k + 1] * vb[s0 + u][2 * j + 1]);
+                                }
+                            }
+                        if (need) {
+                            seen = fresh;
                             long long spins = 0;
-                            while (!stalled) {
-                                seen = lds_counter_load(&cons_s);
-                                if ((int)(seen - target) >= 0) break;
+                            while ((int)(seen - target) < 0 && !stalled) {
                                 if (++spins > kTpSpinLimit) {
                                     stalled = true;
                                     __hip_atomic_store(&abort_s, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
                                     __hip_atomic_store(a.abort_flag, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                                 }
                                 __builtin_amdgcn_s_sleep(1);
+                                seen = lds_counter_load(&cons_s);
                             }
                         }
-                        if (c == 0 && part == 0)
-                            pr[0].x = coef_s[2] * stage[(int)((st - 1) & 1) * NB + r];   // lag 2: V_{st-1}
-                        double2* dst = reinterpret_cast<double2*>(P + (size_t)r * PS + 2 + c * C + part * PER);
+                        if (c0 == 0 && part == 0)
+                            pr[0][0].x = coef_s[2] * stage[(int)((st - 1) & 1) * NB + r];   // lag 2: V_{st-1}
 #pragma unroll
-                        for (int j = 0; j < PER / 2; ++j) dst[j] = pr[j];
+                        for (int u = 0; u < U; ++u)
+                            if (u < cnt) {
+                                double2* dst = reinterpret_cast<double2*>(P + (size_t)r * PS + 2 + (c0 + u) * C + part * PER);
+#pragma unroll
+                                for (int j = 0; j < PER / 2; ++j) dst[j] = pr[u][j];
+                            }
+#pragma unroll
+                        for (int u = 0; u < U; ++u)
+                            if (u < cnt && c0 + u + D - 1 < nchn) ldr(vb[(s0 + u + D - 1) % D], c0 + u + D - 1);
                     };
-                    // chunk q is loaded into vb[q % D], D - 1 chunks ahead of its products
-                    constexpr int D = kTpLoadDepth;
-                    double vb[D][PER];
 #pragma unroll
                     for (int j = 0; j < D - 1; ++j)
                         if (j < nchn) ldr(vb[j], j);
                     int c = 0;
                     for (; c + D <= nchn; c += D) {
 #pragma unroll
-                        for (int j = 0; j < D; ++j) {
-                            if (c + j + D - 1 < nchn) ldr(vb[(j + D - 1) % D], c + j + D - 1);
-                            put(vb[j], c + j);
-                        }
+                        for (int j = 0; j < D; j += U) unit(c + j, j, U);
                     }
 #pragma unroll
-                    for (int j = 0; j < D - 1; ++j)
-                        if (c + j < nchn) put(vb[j], c + j);
+                    for (int j = 0; j < D; j += U)
+                        if (c + j < nchn) unit(c + j, j, min(U, nchn - c - j));
                     if (nchn > 0) {   // the chain reads the lags past limn (last chunk only) as +0.0
 #pragma unroll
                         for (int j = 0; j < PER; ++j) {
