@@ -870,17 +870,26 @@ __device__ __forceinline__ void tp_body(const PersistArgs& a, const int g) {
         // pull is one batch of spike-word and weight reads, then the in-order adds
         unsigned pk_r[kRegEdges];
         double wv_r[kRegEdges];
-        double uin_r[kMaxDinReg];
-        int len_r = 0, len_max = 0;
-        if constexpr (R == 5) {
-            len_r = lane < nloc ? re - rs : 0;
-            int mx = len_r;
-            for (int o = 32; o > 0; o >>= 1) mx = max(mx, __shfl_xor(mx, o));
-            len_max = __builtin_amdgcn_readfirstlane(mx);
+        // inputs of 64 macro steps in the lanes: lane l of uw_cur holds u[uw_base + l],
+        // of uw_nxt u[uw_base + 64 + l], loaded a window ahead (far-memory loads beside
+        // the exchange polls take more than a step)
+        double uw_cur[kMaxDinReg], uw_nxt[kMaxDinReg];
+        long long uw_base = t0;
+        auto uw_load = [&](double* w, long long base) {
+            const long long tl = base + lane;
 #pragma unroll
-            for (int q = 0; q < kRegEdges; ++q) {
+            for (int j = 0; j < kMaxDinReg; ++j) w[j] = (j < a.din && tl < t0 + a.T) ? a.u[tl * a.din + j] : 0.0;
+        };
+        if constexpr (R == 5) {
+            const int len_r = lane < nloc ? re - rs : 0;
+#pragma unroll
+            for (int q = 0; q < kRegEdges; ++q) {   // absent edges: word 0, weight +0.0
                 pk_r[q] = q < len_r ? eidx_s[rs + q] : 0u;
                 wv_r[q] = q < len_r ? eval_s[rs + q] : 0.0;
+            }
+            if (win_reg) {
+                uw_load(uw_cur, t0);
+                uw_load(uw_nxt, t0 + 64);
             }
         }
         for (long long t = 0; t < a.T; ++t) {
@@ -1055,36 +1064,40 @@ __device__ __forceinline__ void tp_body(const PersistArgs& a, const int g) {
                 } else if constexpr (R == 5) {
                     // ---------------- exchange with the row pull from registers (simple
                     // synapses, CSR slice in LDS, rows of at most kRegEdges edges): one batch
-                    // of spike-word and weight reads, then the in-order adds (sparse.c:68-76;
-                    // a silent edge adds +0.0, which leaves the sum unchanged).  The inputs
-                    // of macro step t (uin_r) were loaded during step t-1.
-                    if (t == 0 && mi == 0 && win_reg) {
-#pragma unroll
-                        for (int j = 0; j < kMaxDinReg; ++j) uin_r[j] = (j < a.din) ? a.u[tt * a.din + j] : 0.0;
-                    }
+                    // of spike-word and weight reads, then the in-order adds over all
+                    // kRegEdges slots (sparse.c:68-76; a silent or absent edge adds +0.0, which
+                    // leaves the sum unchanged: the sum starts at +0.0 and is never -0.0).
                     const bool aborted = !gather_spikes(a.xch, nw, NB, wpg, n, st, st0, S_in, bits_s,
                                                         a.abort_flag, &abort_s, lane);
                     __builtin_amdgcn_wave_barrier();   // bits_s written by this wave, read by its lanes
                     if (!aborted) {
                         unsigned bw[kRegEdges];
 #pragma unroll
-                        for (int q = 0; q < kRegEdges; ++q)
-                            if (q < len_max) bw[q] = bits_s[pk_r[q] >> 5];
+                        for (int q = 0; q < kRegEdges; ++q) bw[q] = bits_s[pk_r[q] >> 5];
                         double sum = 0.0;
 #pragma unroll
                         for (int q = 0; q < kRegEdges; ++q)
-                            if (q < len_max) {
-                                const bool hit = q < len_r && ((bw[q] >> (pk_r[q] & 31)) & 1u);
-                                sum = sum + (hit ? wv_r[q] : 0.0);
-                            }
+                            sum = sum + (((bw[q] >> (pk_r[q] & 31)) & 1u) ? wv_r[q] : 0.0);
+                        if (mi == 0 && win_reg && tt - uw_base == 64) {   // next input window
+#pragma unroll
+                            for (int j = 0; j < kMaxDinReg; ++j) uw_cur[j] = uw_nxt[j];
+                            uw_base += 64;
+                            uw_load(uw_nxt, uw_base + 64);
+                        }
                         if (lane < nloc) {
                             rec_s[sb * NB + lane] = sum;
                             if (mi == 0) {
                                 double su = 0.0;
                                 if (win_reg) {
+                                    const int k = (int)(tt - uw_base);   // uniform, 0..63
 #pragma unroll
                                     for (int j = 0; j < kMaxDinReg; ++j)
-                                        if (j < a.din) su = su + win[j] * uin_r[j];
+                                        if (j < a.din) {
+                                            const unsigned long long b = __double_as_longlong(uw_cur[j]);
+                                            const unsigned lo = __builtin_amdgcn_readlane((unsigned)b, k);
+                                            const unsigned hi = __builtin_amdgcn_readlane((unsigned)(b >> 32), k);
+                                            su = su + win[j] * __longlong_as_double(((long long)hi << 32) | lo);
+                                        }
                                 } else {
                                     for (int j = 0; j < a.din; ++j)
                                         su = su + a.W_in[(size_t)i * a.din + j] * a.u[tt * a.din + j];
@@ -1092,10 +1105,6 @@ __device__ __forceinline__ void tp_body(const PersistArgs& a, const int g) {
                                 ext_s[tb * NB + lane] = a.gain * su;
                             }
                         }
-                    }
-                    if (mi == 0 && win_reg && t + 1 < a.T) {   // the next macro step's inputs
-#pragma unroll
-                        for (int j = 0; j < kMaxDinReg; ++j) uin_r[j] = (j < a.din) ? a.u[(tt + 1) * a.din + j] : 0.0;
                     }
                     __syncthreads();
                     if (__builtin_amdgcn_readfirstlane(abort_s)) break;
