@@ -978,10 +978,14 @@ __device__ __forceinline__ void tp_body(const PersistArgs& a, const int g) {
                     double leak = -(Vprev - p[2]) / p[3];   // before the barrier: the wait hides it
                     asm volatile("" : "+v"(leak));          // (kept here, not sunk past the barrier)
                     __syncthreads();   // rec / ext of this step and the products of the next are ready
-                    // the input terms are read together with the abort flag (one LDS round trip)
+                    // the input terms are read together with the abort flag (one LDS round
+                    // trip: the empty statement keeps the compiler from sinking the two reads
+                    // past the abort branch, which would cost a second one)
                     const int ll = lane < NB ? lane : 0;
-                    const double xe = ext_s[tb * NB + ll], xr = rec_s[sb * NB + ll];
-                    if (__builtin_amdgcn_readfirstlane(abort_s)) break;
+                    double xe = ext_s[tb * NB + ll], xr = rec_s[sb * NB + ll];
+                    const int ab = abort_s;
+                    asm volatile("" : "+v"(xe), "+v"(xr));
+                    if (__builtin_amdgcn_readfirstlane(ab)) break;
                     if (own) {                                  // flif_gl.c:50-100
                         const double x = xe + xr;
                         spk = false;
