@@ -487,8 +487,9 @@ __host__ __device__ __forceinline__ int tp_hist_stride(int L) { return (L + 5) |
 // count reads only, which is conservative when a counter store is outstanding,
 // and at most 15 LDS operations are ever in flight.  The last bursts read up to
 // 224 bytes past the row (the next row, or the coefficient table after the last
-// one) and are waited for before the statement ends.  After chunk c (32 lags)
-// the LDS counter at byte address ca is set to cv + c, as the producers expect.
+// one) and are waited for before the statement ends.  Once every read of chunk
+// c (32 lags) has completed, before its last adds, the LDS counter at byte
+// address ca is set to cv + c: the producers may then overwrite the chunk.
 // va is the byte address of P[2] in LDS (16-byte aligned).
 __device__ __forceinline__ double chain_chunks(double h, unsigned va, int nch, unsigned ca, unsigned cv) {
     asm volatile(
@@ -536,6 +537,8 @@ __device__ __forceinline__ double chain_chunks(double h, unsigned va, int nch, u
         "v_add_f64 %[h], %[h], v[92:93]\n\t"
         "v_add_f64 %[h], %[h], v[94:95]\n\t"
         "s_waitcnt lgkmcnt(6)\n\t"
+        "ds_write_b32 %[ca], %[cv]\n\t"
+        "v_add_u32 %[cv], 1, %[cv]\n\t"
         "ds_read_b128 v[152:155], %[va] offset:352\n\t"
         "ds_read_b128 v[156:159], %[va] offset:368\n\t"
         "ds_read_b128 v[160:163], %[va] offset:384\n\t"
@@ -560,8 +563,6 @@ __device__ __forceinline__ double chain_chunks(double h, unsigned va, int nch, u
         "v_add_f64 %[h], %[h], v[122:123]\n\t"
         "v_add_f64 %[h], %[h], v[124:125]\n\t"
         "v_add_f64 %[h], %[h], v[126:127]\n\t"
-        "ds_write_b32 %[ca], %[cv]\n\t"
-        "v_add_u32 %[cv], 1, %[cv]\n\t"
         "s_waitcnt lgkmcnt(6)\n\t"
         "ds_read_b128 v[184:187], %[va] offset:480\n\t"
         "ds_read_b128 v[188:191], %[va] offset:496\n\t"
@@ -588,6 +589,8 @@ __device__ __forceinline__ double chain_chunks(double h, unsigned va, int nch, u
         "v_add_f64 %[h], %[h], v[156:157]\n\t"
         "v_add_f64 %[h], %[h], v[158:159]\n\t"
         "s_waitcnt lgkmcnt(6)\n\t"
+        "ds_write_b32 %[ca], %[cv]\n\t"
+        "v_add_u32 %[cv], 1, %[cv]\n\t"
         "ds_read_b128 v[88:91], %[va] offset:608\n\t"
         "ds_read_b128 v[92:95], %[va] offset:624\n\t"
         "ds_read_b128 v[96:99], %[va] offset:640\n\t"
@@ -612,8 +615,6 @@ __device__ __forceinline__ double chain_chunks(double h, unsigned va, int nch, u
         "v_add_f64 %[h], %[h], v[186:187]\n\t"
         "v_add_f64 %[h], %[h], v[188:189]\n\t"
         "v_add_f64 %[h], %[h], v[190:191]\n\t"
-        "ds_write_b32 %[ca], %[cv]\n\t"
-        "v_add_u32 %[cv], 1, %[cv]\n\t"
         "v_add_u32 %[va], 0x200, %[va]\n\t"
         "s_sub_u32 %[n], %[n], 2\n\t"
         "s_branch L_chain_loop_%=\n\t"
@@ -646,6 +647,8 @@ __device__ __forceinline__ double chain_chunks(double h, unsigned va, int nch, u
         "v_add_f64 %[h], %[h], v[92:93]\n\t"
         "v_add_f64 %[h], %[h], v[94:95]\n\t"
         "s_waitcnt lgkmcnt(6)\n\t"
+        "ds_write_b32 %[ca], %[cv]\n\t"
+        "v_add_u32 %[cv], 1, %[cv]\n\t"
         "ds_read_b128 v[152:155], %[va] offset:352\n\t"
         "ds_read_b128 v[156:159], %[va] offset:368\n\t"
         "ds_read_b128 v[160:163], %[va] offset:384\n\t"
@@ -670,8 +673,6 @@ __device__ __forceinline__ double chain_chunks(double h, unsigned va, int nch, u
         "v_add_f64 %[h], %[h], v[122:123]\n\t"
         "v_add_f64 %[h], %[h], v[124:125]\n\t"
         "v_add_f64 %[h], %[h], v[126:127]\n\t"
-        "ds_write_b32 %[ca], %[cv]\n\t"
-        "v_add_u32 %[cv], 1, %[cv]\n\t"
         "L_chain_end_%=:\n\t"
         "s_waitcnt lgkmcnt(0)\n\t"
         : [h] "+v"(h), [va] "+v"(va), [n] "+s"(nch), [cv] "+v"(cv)
