@@ -999,24 +999,32 @@ __device__ __forceinline__ void tp_body(const PersistArgs& a, const int g) {
                         }
                     }
                     publish_spikes(a.xch, nw, g, wpg, st, lane, __ballot(own && spk));
+                    // LDS first (stage, history row, the producers' signal), then the HBM
+                    // stores: the next chain's first LDS wait then finds the writes done.
+                    // Lane 0 (always an owner) signals after the writes of every owner lane:
+                    // a wave's LDS instructions complete in issue order, and the empty
+                    // statement keeps the compiler from moving the signal above them.
                     if (own) {
-                        a.ring[(size_t)head * n + i] = V;
                         stage[sb * NB + lane] = V;
                         if (HL) {
                             const int e = L - 1 - head;
                             hist_s[lane * HR + e] = V;
                             for (int q = e; q < HR - L; q += L) hist_s[lane * HR + L + q] = V;
                         }
+                        asm volatile("" ::: "memory");   // V_st staged before the signal
+                        if (lane == 0) {
+                            if constexpr (!WIN) lds_counter_store(&cons_s, (m + 1u) * 64u);   // tail done
+                            else lds_counter_store(&tail_s, m + 1u);
+                        }
+                    }
+                    asm volatile("" ::: "memory");
+                    if (own) {
+                        a.ring[(size_t)head * n + i] = V;
                         tp = tp + a.dt;
                         if (mi == a.micro - 1) {
                             if (rec_V) rec_V[(size_t)t * n + i] = V;
                             if (rec_S) rec_S[(size_t)t * n + i] = spk ? 1 : 0;
                         }
-                    }
-                    if constexpr (!WIN) lds_counter_store(&cons_s, (m + 1u) * 64u);   // tail done: V_st staged
-                    else {
-                        asm volatile("" ::: "memory");   // V_st staged before the signal
-                        lds_counter_store(&tail_s, m + 1u);
                     }
                 } else if constexpr (R == 4) {
                     // ---------------- exchange, row pull, external input
