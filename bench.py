@@ -254,13 +254,14 @@ def run_batched(args):
     """64 FLIF-GL reservoirs of uneven N (log-U[256, 8192]), L, alpha
     and topology (ER p=0.05 / BA m=5), N(0,1) weights rescaled to rho, on one
     13-channel input of AR(1) class sequences, advanced in time blocks of K
-    steps captured once in a HIP graph and replayed every block.  By default
-    all 64 reservoirs run in one cooperative launch of 64-neuron slices per
-    block (TB_ONE_LAUNCH); with --packed the reservoirs that fit the
-    product-wave kernel run as launches packed per neurons-per-workgroup class
-    (at most one workgroup per CU) and the others one launch each per block.
-    K is picked by timing one block per candidate; the product-wave launches
-    are also timed eagerly for comparison."""
+    steps captured once in a HIP graph and replayed every block
+    (spires_amd.BlockRunner).  The tile shape -- all 64 reservoirs in one
+    cooperative launch of 64-neuron one-wave slices per block, or product-wave
+    workgroups of 16/32 neurons packed per class into launches of at most one
+    workgroup per CU (the others one launch each per block) -- and K are picked
+    by spires_amd.autotune_blocks, which times one block per candidate; --packed
+    keeps the product-wave shape.  The product-wave launches are also timed
+    eagerly for comparison."""
     import numpy as np
     import spires_amd as sa
     from spires_amd import workloads
@@ -285,71 +286,55 @@ def run_batched(args):
     singles = [i for i in range(len(res)) if plans[i]["batch_nb"] == 0]
     total_n = sum(sp["n"] for sp in specs)
     flags = sa.REC_STATE | sa.REC_APPEND
-    # the graph's reservoirs and flags: the product-wave launches, or (one launch)
-    # every reservoir as 64-neuron one-wave slices of a single launch
-    one_launch = not args.packed
-    gsel, gflags, gsingles = (list(range(len(res))), sa.REC_STATE | sa.TB_ONE_LAUNCH, []) if one_launch \
-        else (tp, sa.REC_STATE, singles)
 
     def eager_block(t, K):
-        """The same launches without the graph (run_resident_batch per launch)."""
-        cur, wgs = [], 0
+        """The product-wave launches without the graph (run_resident_batch per launch)."""
         groups = []
         for nb in (32, 16):
-            cur, wgs = [], 0
+            cur = []
             for i in tp:
                 if plans[i]["batch_nb"] != nb:
                     continue
-                w = -(-specs[i]["n"] // nb)
                 if cur and sa.batch_workgroups([res[j] for j in cur + [i]]) == 0:
                     groups.append(cur)
-                    cur, wgs = [], 0
+                    cur = []
                 cur.append(i)
-                wgs += w
             if cur:
                 groups.append(cur)
         for g in groups:
             sa.run_resident_batch([res[i] for i in g], t, K, flags)
-
-    def single_block(t, K, which=None):
-        for i in (gsingles if which is None else which):
+        for i in singles:
             res[i].run_resident(t, K, flags)
+        sa.synchronize(res)
 
     # untimed warm-up until every history is full
     t = 1024
-    tb = sa.TimeBlock([res[i] for i in gsel], t, gflags)
-    tb.run(1)
-    tb.close()
-    single_block(0, t)
-    # K autotune: one graph block per candidate (each advances every reservoir)
-    ktimes = {}
-    for K in (64, 256, 1024):
-        tb = sa.TimeBlock([res[i] for i in gsel], K, gflags)
-        t0 = time.perf_counter()
-        tb.run(1)
-        single_block(t, K)
-        ktimes[K] = (time.perf_counter() - t0) / K * 1e6
-        tb.close()
-        t += K
-    K = min(ktimes, key=ktimes.get)
-    # eager launches for comparison: two blocks
+    br = sa.BlockRunner(res, t, "slices")
+    br.run(0)
+    br.close()
+    sa.synchronize(res)
+    # autotune: the tile shape (one launch of 64-neuron one-wave slices, or packed
+    # product-wave workgroups) and K, one block per candidate (each advances every reservoir)
+    shapes = ("products",) if args.packed else sa.BlockRunner.SHAPES
+    shape, K, table, t = sa.autotune_blocks(res, t, Ks=(64, 256, 1024), shapes=shapes)
+    # eager product-wave launches for comparison: two blocks
     t0 = time.perf_counter()
     for _ in range(2):
         eager_block(t, K)
-        single_block(t, K, singles)
         t += K
     eager_us = (time.perf_counter() - t0) / (2 * K) * 1e6
     # timed: graph replays of whole blocks
-    tb = sa.TimeBlock([res[i] for i in gsel], K, gflags)
-    launches = tb.launches + len(gsingles)
+    br = sa.BlockRunner(res, K, shape)
+    launches = br.launches
+    gsingles = [i for i in range(len(res)) if shape == "products" and plans[i]["batch_nb"] == 0]
     blocks = (T - t) // K
     t0 = time.perf_counter()
-    for _ in range(blocks):
-        tb.run(1)
-        single_block(t, K)
-        t += K
+    br.run(t, blocks)
+    sa.synchronize(res)
     wall = time.perf_counter() - t0
-    tb.close()
+    t += blocks * K
+    br.close()
+    one_launch = shape == "slices"
     steps = blocks * K
     value = total_n * steps / wall
     alg = sum(b * sp["n"] for b, sp in zip(bps, specs)) * steps / wall / 1e9
@@ -362,10 +347,10 @@ def run_batched(args):
         "config": {"workload": "64 FLIF-GL reservoirs, N log-U[256,8192] (total %d neurons), ER p=0.05 / BA m=5, "
                                "alpha U[0.5,1], T_mem in {128,256,512,1024}, rho U[0.7,1.2], N(0,1) weights; "
                                "13-channel AR(1) class sequences (512 sequences, %d frames)" % (total_n, T),
-                   "block_k": K,
-                   "graph": ("HIP graph per time block: one launch of 64-neuron slices (all %d reservoirs)" % len(gsel))
+                   "block_k": K, "tile_shape": shape,
+                   "graph": ("HIP graph per time block: one launch of 64-neuron slices (all %d reservoirs)" % len(res))
                    if one_launch else "HIP graph per time block (%d product-wave reservoirs)" % len(tp)},
-        "k_autotune_us_per_step": ktimes,
+        "autotune_us_per_step": {"%s/K=%d" % k: v for k, v in table.items()},
         "eager_us_per_step": eager_us,
         "launches_per_block": launches,
         "single_launches": [dict(n=specs[i]["n"], T_mem=specs[i]["T_mem"], kernel=plans[i]["kernel"]) for i in gsingles],

@@ -385,6 +385,81 @@ class TimeBlock:
     __del__ = close
 
 
+class BlockRunner:
+    """Advances exact-mode FLIF-GL reservoirs together in time blocks of K steps,
+    each block one replay of a HIP graph (TimeBlock), in one of two tile shapes:
+
+      "slices"   every reservoir in one cooperative launch of 64-neuron one-wave
+                 slices (TB_ONE_LAUNCH, k_mux_gl);
+      "products" workgroups of 16 or 32 neurons with product waves, packed per
+                 neurons-per-workgroup class into launches of at most one
+                 workgroup per CU (k_persist_tp_batch); a reservoir whose product
+                 rows do not fit in LDS runs as a launch of its own per block.
+
+    Inputs are read and records written at row = macro step since reset; reserve
+    the records (`flags`: REC_STATE and/or REC_RASTER) before creating it.
+    run(t, blocks) advances every reservoir from step t by blocks * K steps."""
+
+    SHAPES = ("slices", "products")
+
+    def __init__(self, reservoirs, K, shape="slices", flags=REC_STATE):
+        if shape not in self.SHAPES:
+            raise ValueError(f"tile shape {shape!r} not in {self.SHAPES}")
+        self.res, self.K, self.shape, self.flags = list(reservoirs), int(K), shape, flags
+        if shape == "slices":
+            graph, self.singles = self.res, []
+        else:
+            plans = [r.kernel_plan() for r in self.res]
+            graph = [r for r, p in zip(self.res, plans) if p["batch_nb"] > 0]
+            self.singles = [r for r, p in zip(self.res, plans) if p["batch_nb"] == 0]
+        self.tb = TimeBlock(graph, self.K, flags | (TB_ONE_LAUNCH if shape == "slices" else 0)) if graph else None
+
+    @property
+    def launches(self):
+        return (self.tb.launches if self.tb else 0) + len(self.singles)
+
+    def run(self, t, blocks=1):
+        for b in range(blocks):
+            if self.tb:
+                self.tb.run(1)
+            for r in self.singles:
+                r.run_resident(t + b * self.K, self.K, self.flags | REC_APPEND)
+
+    def close(self):
+        if self.tb:
+            self.tb.close()
+            self.tb = None
+
+
+def autotune_blocks(reservoirs, t, Ks=(64, 256, 1024), shapes=BlockRunner.SHAPES, flags=REC_STATE):
+    """Picks the tile shape and block length K for BlockRunner by timing one
+    block of each candidate from step t on (the blocks advance the reservoirs,
+    so no step is run twice).  Returns (shape, K, {(shape, K): us per step}, t
+    after the timed blocks)."""
+    import time as _time
+    table = {}
+    for shape in shapes:
+        for K in Ks:
+            br = BlockRunner(reservoirs, K, shape, flags)
+            try:
+                synchronize(reservoirs)
+                t0 = _time.perf_counter()
+                br.run(t)
+                synchronize(reservoirs)
+                table[(shape, K)] = (_time.perf_counter() - t0) / K * 1e6
+            finally:
+                br.close()
+            t += K
+    shape, K = min(table, key=table.get)
+    return shape, K, table, t
+
+
+def synchronize(reservoirs):
+    """Waits for the device work queued for these reservoirs."""
+    for r in reservoirs:
+        r.sync()
+
+
 class Surrogate:
     """The AGILE surrogate MLP in_dim -> hidden -> hidden -> 1 on the GPU
     (spires_ext_mlp_*): forward, backward and Adam run as HIP kernels."""

@@ -135,3 +135,27 @@ def test_one_launch_time_block_long_histories_and_psc():
     tb.close()
     for r in A + B:
         r.close()
+
+
+def test_autotuned_block_runner_matches_eager_runs():
+    """spires_amd.autotune_blocks times one block per (tile shape, K) candidate,
+    each advancing every reservoir, and BlockRunner then continues in the shape
+    it picked: the records of all these blocks equal eager batched runs of the
+    same steps bit for bit."""
+    print("case: autotuned block runner, 2 shapes x K in (8, 16), then 3 blocks", flush=True)
+    us = [np.random.default_rng(60 + k).uniform(0, 0.5, (T_ALL, 1)) for k in range(len(SPECS))]
+    A = [_make(s, u) for s, u in zip(SPECS, us)]     # block runner
+    B = [_make(s, u) for s, u in zip(SPECS, us)]     # eager
+    shape, K, table, t = sa.autotune_blocks(A, 0, Ks=(8, 16), flags=FLAGS)
+    assert set(table) == {(s, k) for s in sa.BlockRunner.SHAPES for k in (8, 16)}
+    assert t == 48 and shape in sa.BlockRunner.SHAPES and K in (8, 16)
+    br = sa.BlockRunner(A, K, shape, FLAGS)
+    br.run(t, 3)
+    br.close()
+    sa.synchronize(A)
+    rows = t + 3 * K
+    sa.run_resident_batch(B, 0, rows, FLAGS | sa.REC_APPEND)
+    for a, b in zip(A, B):
+        assert _same(a, b, rows)
+    for r in A + B:
+        r.close()
