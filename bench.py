@@ -220,7 +220,7 @@ def run_agile(args):
         "surrogate": runs,
         "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": gbs / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_persist_tp_batch (plain search launches, algorithmic bytes / launch time)"},
+                     "kernel": "k_mux_gl (plain search: 32 trials per launch of 64-neuron slices, 1000-step blocks; algorithmic bytes / run time)"},
     }
     if not args.no_cpu:
         line["cpu_baseline"] = agile_cpu_baseline()

@@ -110,9 +110,19 @@ class Evaluator:
     them to a fidelity on the GPU: batched runs, batched ridge fits, readout
     predictions on the device."""
 
-    def __init__(self, n=2048, f_max=18000, seed=0, workers=8):
+    def __init__(self, n=2048, f_max=18000, seed=0, workers=8, runner="auto", block=1000, slices_min=16):
+        """runner: "batch" runs trials that start at the same step as packed
+        product-wave launches (run_resident_batch, at most one workgroup per
+        CU); "slices" runs them all in one launch of 64-neuron one-wave slices
+        per block of at most `block` steps (spires_amd.BlockRunner; the largest
+        length that divides the group's steps); "auto" takes the slices for groups of at least
+        `slices_min` trials (a slices launch lasts as long as its slowest trial
+        and suits large groups) and packed launches for smaller ones."""
         self.n = n
         self.f_max = f_max
+        self.runner = runner
+        self.block = block
+        self.slices_min = slices_min
         series = workloads.mackey_glass(f_max + HORIZON + 1, seed=seed)
         self.u = np.ascontiguousarray(series[:f_max].reshape(f_max, 1))
         self.y = np.ascontiguousarray(series[HORIZON:f_max + HORIZON])
@@ -175,6 +185,8 @@ class Evaluator:
         rp, ci, val, W_in = self._weights(th, trial_seed)
         r = sa.Reservoir(cfg, csr=(rp, ci, val), W_in=W_in)
         r.upload_inputs(self.u)
+        if self.runner in ("slices", "auto"):   # time blocks write into reserved records
+            r.reserve_records(self.f_max, sa.REC_STATE)
         r.bps = workloads.algorithmic_bytes_per_neuron_step(self.n, len(ci), int(th[4]))
         return r
 
@@ -231,9 +243,28 @@ class Evaluator:
         trials = list(trials)
         self._ensure(trials)
         t0 = time.perf_counter()
-        for done, grp in self._launch_groups([t for t in trials if t.done < F]):
+        by_done = {}
+        for t in trials:
+            if t.done < F:
+                by_done.setdefault(t.done, []).append(t)
+        groups = []
+        for done, grp in sorted(by_done.items()):
+            if self.runner == "slices" or (self.runner == "auto" and len(grp) >= self.slices_min):
+                groups.append((done, grp, True))
+            else:
+                groups += [(d, g, False) for d, g in self._launch_groups(grp)]
+        for done, grp, slices in groups:
             steps = F - done
-            if len(grp) > 1:
+            if slices:
+                K = min(self.block, steps)   # the largest block length that divides the steps
+                while steps % K:
+                    K -= 1
+                br = sa.BlockRunner([t.res for t in grp], K, "slices")
+                try:
+                    br.run(done, steps // K)
+                finally:
+                    br.close()
+            elif len(grp) > 1:
                 sa.run_resident_batch([t.res for t in grp], done, steps, sa.REC_STATE | sa.REC_APPEND)
             else:
                 grp[0].res.run_resident(done, steps, sa.REC_STATE | sa.REC_APPEND)
