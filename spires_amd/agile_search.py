@@ -110,7 +110,7 @@ class Evaluator:
     them to a fidelity on the GPU: batched runs, batched ridge fits, readout
     predictions on the device."""
 
-    def __init__(self, n=2048, f_max=18000, seed=0, workers=8, runner="auto", block=1000, slices_min=16):
+    def __init__(self, n=2048, f_max=18000, seed=0, workers=16, runner="auto", block=1000, slices_min=16):
         """runner: "batch" runs trials that start at the same step as packed
         product-wave launches (run_resident_batch, at most one workgroup per
         CU); "slices" runs them all in one launch of 64-neuron one-wave slices
