@@ -162,14 +162,24 @@ class Evaluator:
         total = n * n
         inv = 1.0 / math.log1p(-c)
 
-        def gaps(k):   # Geometric(c) on {1, 2, ...} by inversion
-            return (np.log(1.0 - rng.random(k)) * inv).astype(np.int64) + 1
+        def gaps(k):   # Geometric(c) on {1, 2, ...} by inversion: floor(log(1 - U) / log(1 - c)) + 1
+            g = rng.random(k)
+            np.subtract(1.0, g, out=g)
+            np.log(g, out=g)
+            g *= inv
+            out = g.astype(np.int64)
+            out += 1
+            return out
 
-        pos = np.cumsum(gaps(int(total * c * 1.1) + 64)) - 1
+        pos = np.cumsum(gaps(int(total * c * 1.1) + 64))
+        pos -= 1
         while pos[-1] < total:
             pos = np.concatenate([pos, np.cumsum(gaps(int(total * c * 0.1) + 64)) + pos[-1]])
-        pos = pos[pos < total]
-        rows, cols = np.divmod(pos, n)
+        pos = pos[:np.searchsorted(pos, total)]   # increasing
+        if n & (n - 1) == 0:
+            rows, cols = pos >> (n.bit_length() - 1), pos & (n - 1)
+        else:
+            rows, cols = np.divmod(pos, n)
         keep = rows != cols
         rows, cols = rows[keep], cols[keep]
         rp = np.zeros(n + 1, np.int64)
